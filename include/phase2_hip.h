@@ -1,0 +1,142 @@
+#ifndef PHASE2_HIP_H
+#define PHASE2_HIP_H
+
+/*
+ * Thin C ABI between the C host library and the hand-written gfx950 HIP
+ * kernels (phase2_amd/csrc/kernels.hip).  Plain pointers and sizes only;
+ * device pointers are opaque `void *`.  Every function returns 0 on success
+ * and -1 on failure, with the reason in ph2hip_strerror().  All work is
+ * ordered on one HIP stream per process (default priority); functions that
+ * return host data synchronise that stream.
+ *
+ * Amplitudes are complex128, interleaved (re, im), 16 B each; a register of
+ * nqb qubits is 2^nqb amplitudes, indexed by the basis state with qubit k
+ * at bit k.
+ *
+ * Rotation terms arrive with c = cos(phi) and s = sin(phi) computed by the
+ * host libm, as the reference CPU backend does (phase2/qreg_qreg.c:189), so
+ * every kernel performs the same IEEE operations as
+ * phase2/qreg_qreg.c:149-160: per component round(round(c*x) +- round(s*y)),
+ * no fused multiply-add.  All kernels are FP64.
+ */
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* -- device and memory --------------------------------------------------- */
+
+/* Select the device and create the stream (idempotent).  Replaces the
+ * device selection of phase2/world_cuda.c:21-43. */
+int ph2hip_init(int device);
+void ph2hip_shutdown(void);
+const char *ph2hip_strerror(void);
+int ph2hip_synchronize(void);
+
+/* Replace cudaMalloc/cudaFree of phase2/qreg_cuda.c:34-37 (one buffer, no
+ * exchange buffer). */
+int ph2hip_malloc(void **dptr, size_t bytes);
+int ph2hip_free(void *dptr);
+int ph2hip_h2d(void *dst, const void *src, size_t bytes);
+int ph2hip_d2h(void *dst, const void *src, size_t bytes);
+int ph2hip_d2d(void *dst, const void *src, size_t bytes);
+
+/* Zero namp amplitudes with a fill kernel (phase2/qreg_cuda.c:96). */
+int ph2hip_zero(void *amp, uint64_t namp);
+/* Single-amplitude write/read (phase2/qreg_cuda.c:70,85). */
+int ph2hip_set1(void *amp, uint64_t i, double re, double im);
+int ph2hip_get1(const void *amp, uint64_t i, double *re_im);
+
+/* -- rotations ------------------------------------------------------------ */
+
+/* One Pauli rotation exp(i phi P): flip mask x, phase mask z, c = cos(phi),
+ * s = sin(phi). */
+struct ph2hip_rot {
+	uint64_t x, z;
+	double c, s;
+};
+
+/* Plain sweep: one HBM pass (one kernel launch) per term, in order.
+ * Replaces kernelPauliRot/kernelMix/kernelAdd of
+ * phase2/qreg_cuda_lo.cu:34-131 on one GPU. */
+int ph2hip_rotate_plain(void *amp, uint32_t nqb, const struct ph2hip_rot *rot,
+	size_t n);
+
+/* Fused sweep: the terms are applied in order, pass by pass; pass p covers
+ * the next pass_len[p] terms and runs as one HBM pass over LDS tiles of
+ * 2^tile_bits[p] amplitudes, each tile a coset of a subspace that contains
+ * the low `low_bits` index bits and every flip mask of the pass.  The state
+ * is bit-identical to ph2hip_rotate_plain.  Fails (-1, nothing applied) if a
+ * pass does not fit its tile (span too large), tile_bits is outside
+ * [PH2HIP_TILE_MIN, PH2HIP_TILE_MAX] or above nqb, or a pass holds more than
+ * PH2HIP_PASS_MAX terms.  No counterpart in the reference. */
+#define PH2HIP_TILE_MIN 9
+#define PH2HIP_TILE_MAX 13
+#define PH2HIP_PASS_MAX 64
+int ph2hip_rotate_fused(void *amp, uint32_t nqb, const struct ph2hip_rot *rot,
+	size_t n, const uint32_t *pass_len, const uint32_t *tile_bits,
+	size_t npass, uint32_t low_bits);
+
+/* Dimension of span{x_k & ~(2^low_bits - 1)} over GF(2) for k in [0, n):
+ * the number of tile bits beyond low_bits a pass over these terms needs. */
+uint32_t ph2hip_span_rank(const struct ph2hip_rot *rot, size_t n,
+	uint32_t low_bits);
+
+/* -- overlap and Slater-Condon -------------------------------------------- */
+
+/* out[2k], out[2k+1] = amp[idx[k]] (host arrays; idx < 2^nqb checked).
+ * Replaces the per-determinant cudaMemcpy of phase2/qreg_cuda.c:66-74. */
+int ph2hip_gather(const void *amp, uint32_t nqb, const uint64_t *idx,
+	size_t n, double *out);
+
+/* Device-resident Slater-Condon tables: occupation masks and determinants
+ * of the alpha and beta k-subsets (host arrays copied to the device). */
+struct ph2hip_sc;
+int ph2hip_sc_create(struct ph2hip_sc **sc, const uint64_t *mask_a,
+	size_t Ma, const uint64_t *mask_b, size_t Mb);
+int ph2hip_sc_set_dets(struct ph2hip_sc *sc, const double *det_a,
+	const double *det_b);
+void ph2hip_sc_destroy(struct ph2hip_sc *sc);
+
+/* amp[idx(a,b)] = (c, 0) or amp[idx(a,b)].re += c for every pair with
+ * c = weight*det_a[a]*det_b[b], |c| >= 1e-12 (phase2/state_prep_coeff.c:
+ * 181-222).  Fails if any index would exceed 2^nqb. */
+int ph2hip_sc_scatter(void *amp, uint32_t nqb, const struct ph2hip_sc *sc,
+	uint32_t n_sites, double weight, int tapered, int accumulate);
+
+/* out = sum_{a,b} c(a,b) * amp[idx(a,b)] as (re, im), FP64, reduced in a
+ * fixed tree order (phase2/state_prep_coeff.c:224-274). */
+int ph2hip_sc_inner(const void *amp, uint32_t nqb, const struct ph2hip_sc *sc,
+	uint32_t n_sites, double weight, int tapered, double *out);
+
+/* -- measurement ----------------------------------------------------------- */
+
+/* Per-launch timing of the rotation kernels: while enabled, every launch of
+ * ph2hip_rotate_plain / ph2hip_rotate_fused is bracketed by HIP events on
+ * the library stream; ph2hip_prof_read synchronises and returns launch
+ * counts and summed kernel milliseconds per kind since the last reset.
+ * Kinds: 0 plain pair pass, 1 plain diagonal pass, 2 fused pass. */
+#define PH2HIP_PROF_KINDS 3
+struct ph2hip_prof {
+	uint64_t launches[PH2HIP_PROF_KINDS];
+	double ms[PH2HIP_PROF_KINDS];
+	uint64_t terms[PH2HIP_PROF_KINDS];
+};
+int ph2hip_prof_enable(int on);
+int ph2hip_prof_read(struct ph2hip_prof *out, int reset);
+
+/* Events on the library stream for caller-side timing. */
+int ph2hip_mark(int slot);
+int ph2hip_elapsed_ms(int slot_start, int slot_stop, double *ms);
+
+/* sum_i |amp[i]|^2 (norm-drift reporting), fixed reduction order. */
+int ph2hip_norm2(const void *amp, uint32_t nqb, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHASE2_HIP_H */

@@ -1,0 +1,27 @@
+#ifndef PHASE2_AMD_INTERNAL_H
+#define PHASE2_AMD_INTERNAL_H
+
+/* Helpers shared by the library sources (not installed). */
+
+#include <float.h>
+#include <math.h>
+#include <stddef.h>
+
+#include "phase2/circ.h"
+
+typedef int (*term_cmp)(const struct circ_hamil_term *,
+	const struct circ_hamil_term *);
+
+/* Stable sort of a term array (circ.c). */
+int circ_terms_sort(struct circ_hamil_term *t, size_t n, term_cmp cmp);
+
+/* Sign with a DBL_EPSILON dead zone: -1, +1, or 0 for |a| < DBL_EPSILON
+ * (behaviour of circ/internal.h:26-32 of the reference). */
+static inline double signof(double a)
+{
+	if (fabs(a) < DBL_EPSILON)
+		return 0.0;
+	return a < 0.0 ? -1.0 : 1.0;
+}
+
+#endif /* PHASE2_AMD_INTERNAL_H */

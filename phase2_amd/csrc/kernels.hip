@@ -1,0 +1,1043 @@
+// kernels.hip -- hand-written gfx950 kernels of the phase2 state-vector hot
+// path and the thin C ABI that launches them (declared in include/phase2_hip.h).
+//
+// Register layout: 2^nqb complex128 amplitudes, interleaved (re, im), 16 B
+// each, in one hipMalloc buffer; basis index i has qubit k at bit k.
+//
+// Arithmetic contract (bit identity with the reference CPU backend,
+// phase2/qreg_qreg.c:149-160): built with -ffp-contract=off and every
+// product/sum goes through mul()/add() below, so each component update is
+// round(round(c*x) + round(+-s*y)) with c, s computed by the host libm.
+// Negations and component swaps are exact.  Nothing here is FMA-contracted.
+//
+// Kernels:
+//   k_rot_pair    plain pass, one thread per amplitude pair (i, i^x), i<j
+//   k_rot_diag    plain pass for diagonal terms (x == 0)
+//   k_rot_fused   one HBM pass applying a run of terms in an LDS tile that is
+//                 a coset of span{low bits, flip masks of the run}
+//   k_fill        zeroing (no memset nodes, so the call is graph-safe)
+//   k_gather      amp[idx[k]] for the multidet overlap
+//   k_sc_scatter  Slater-Condon coefficients into the register
+//   k_sc_inner    Slater-Condon inner product, wavefront-shuffle reduction
+//   k_norm2       sum |a|^2, same reduction
+//   k_reduce2     second stage of the two-stage reductions (fixed order)
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <cstdarg>
+
+#include "phase2_hip.h"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// error state and stream
+// ---------------------------------------------------------------------------
+
+char g_err[256] = "";
+hipStream_t g_stream = nullptr;
+int g_device = -1;
+double *g_red = nullptr;       // reduction partials (device)
+size_t g_red_cap = 0;
+
+int fail(const char *fmt, ...)
+{
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(g_err, sizeof g_err, fmt, ap);
+	va_end(ap);
+	return -1;
+}
+
+#define HIPCHK(call)                                                        \
+	do {                                                                \
+		hipError_t e_ = (call);                                     \
+		if (e_ != hipSuccess)                                       \
+			return fail("%s: %s", #call, hipGetErrorString(e_)); \
+	} while (0)
+
+int ensure_init()
+{
+	if (g_stream)
+		return 0;
+	return ph2hip_init(-1);
+}
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double mul(double a, double b)
+{
+	return __dmul_rn(a, b);
+}
+
+__device__ __forceinline__ double add(double a, double b)
+{
+	return __dadd_rn(a, b);
+}
+
+__device__ __forceinline__ uint64_t insert_zero(uint64_t t, uint32_t h)
+{
+	const uint64_t lo = t & ((UINT64_C(1) << h) - 1);
+	return ((t >> h) << (h + 1)) | lo;
+}
+
+__device__ __forceinline__ uint32_t insert_zero32(uint32_t t, uint32_t h)
+{
+	const uint32_t lo = t & ((1u << h) - 1);
+	return ((t >> h) << (h + 1)) | lo;
+}
+
+// Update of one pair (a_i, a_j), i < j, P|i> = i^r4 |j>:
+//   a_i <- c a_i + i conj(s i^r4) a_j,   a_j <- c a_j + i (s i^r4) a_i.
+// For odd r4 the partner enters as itself, for even r4 as i*partner
+// = (-im, re); the sign of s is - for r4 in {2,3} (a_i) and {1,2} (a_j).
+__device__ __forceinline__ void rot_pair(double2 &x, double2 &y, int r4,
+	double c, double s)
+{
+	const bool odd = r4 & 1;
+	const double si = (r4 & 2) ? -s : s;
+	const double sj = ((r4 + 1) & 2) ? -s : s;
+	const double pr = odd ? y.x : -y.y, pi = odd ? y.y : y.x;
+	const double qr = odd ? x.x : -x.y, qi = odd ? x.y : x.x;
+	const double2 nx = make_double2(add(mul(c, x.x), mul(si, pr)),
+		add(mul(c, x.y), mul(si, pi)));
+	const double2 ny = make_double2(add(mul(c, y.x), mul(sj, qr)),
+		add(mul(c, y.y), mul(sj, qi)));
+	x = nx;
+	y = ny;
+}
+
+// Diagonal term (x == 0): a <- c a + i (s z) a with z = (-1)^parity.
+__device__ __forceinline__ void rot_diag(double2 &a, int odd_parity,
+	double c, double s)
+{
+	const double sj = odd_parity ? -s : s;
+	a = make_double2(add(mul(c, a.x), mul(sj, -a.y)),
+		add(mul(c, a.y), mul(sj, a.x)));
+}
+
+constexpr int kBlock = 256;
+
+// ---------------------------------------------------------------------------
+// plain pass
+// ---------------------------------------------------------------------------
+
+// Pair t -> i = t with a zero inserted at bit h (the top bit of x), j = i^x
+// > i.  Consecutive lanes take consecutive t, so both streams coalesce.
+template <int UNR>
+__global__ __launch_bounds__(kBlock) void k_rot_pair(
+	double2 *__restrict__ a, uint64_t npairs, uint64_t x, uint64_t z,
+	uint32_t h, uint32_t y, double c, double s)
+{
+	const uint64_t t0 = (uint64_t)blockIdx.x * (kBlock * UNR) + threadIdx.x;
+	double2 vi[UNR], vj[UNR];
+	uint64_t ii[UNR];
+#pragma unroll
+	for (int u = 0; u < UNR; u++) {
+		const uint64_t t = t0 + (uint64_t)u * kBlock;
+		ii[u] = insert_zero(t, h);
+		if (t < npairs) {
+			vi[u] = a[ii[u]];
+			vj[u] = a[ii[u] ^ x];
+		}
+	}
+#pragma unroll
+	for (int u = 0; u < UNR; u++) {
+		const uint64_t t = t0 + (uint64_t)u * kBlock;
+		if (t < npairs) {
+			const int par = __popcll(ii[u] & z) & 1;
+			rot_pair(vi[u], vj[u], (int)((y + 2 * par) & 3), c, s);
+			a[ii[u]] = vi[u];
+			a[ii[u] ^ x] = vj[u];
+		}
+	}
+}
+
+template <int UNR>
+__global__ __launch_bounds__(kBlock) void k_rot_diag(
+	double2 *__restrict__ a, uint64_t namp, uint64_t z, double c, double s)
+{
+	const uint64_t i0 = (uint64_t)blockIdx.x * (kBlock * UNR) + threadIdx.x;
+	double2 v[UNR];
+#pragma unroll
+	for (int u = 0; u < UNR; u++) {
+		const uint64_t i = i0 + (uint64_t)u * kBlock;
+		if (i < namp)
+			v[u] = a[i];
+	}
+#pragma unroll
+	for (int u = 0; u < UNR; u++) {
+		const uint64_t i = i0 + (uint64_t)u * kBlock;
+		if (i < namp) {
+			rot_diag(v[u], __popcll(i & z) & 1, c, s);
+			a[i] = v[u];
+		}
+	}
+}
+
+// ---------------------------------------------------------------------------
+// fused pass
+// ---------------------------------------------------------------------------
+//
+// A tile is the coset  base ^ span{e_0..e_{L-1}, w_0..w_{g-1}}  of 2^TB
+// amplitudes (TB = L + g).  The w_b have zero low L bits, distinct leading
+// bits p_b (ascending in b) and zeros at every other p_b'.  Tile coordinate
+// v: bits [0,L) are index bits 0..L-1, bit L+b selects w_b.  The tile
+// index T spreads over the index bits that are neither low nor pivots.
+// In these coordinates a term with flip mask x has xt, and
+//   parity(i & z) = parity(base & z) ^ parity(v & zt),
+// and i < i^x iff v < v^xt, so the plain update runs unchanged on v.
+
+struct FusedTerm {
+	uint64_t z;    // physical phase mask (for the per-tile parity)
+	uint32_t xt;   // flip mask in tile coordinates
+	uint32_t zt;   // phase mask in tile coordinates
+	double c, s;
+	uint32_t y;    // popcount(x & z) & 3
+	uint32_t h;    // top bit of xt (unused when xt == 0)
+};
+
+struct FusedPass {
+	uint64_t w[PH2HIP_TILE_MAX];   // gathered basis vectors
+	uint32_t piv[PH2HIP_TILE_MAX]; // their leading bits, ascending
+	uint32_t L, g, nterms, pad;
+	FusedTerm t[PH2HIP_PASS_MAX];
+};
+
+static_assert(sizeof(FusedPass) <= 4000, "kernel argument too large");
+
+template <int TB>
+__global__ __launch_bounds__(kBlock) void k_rot_fused(
+	double2 *__restrict__ a, const FusedPass P)
+{
+	constexpr uint32_t N = 1u << TB;
+	constexpr uint32_t PER = N / kBlock;          // elements per thread
+	__shared__ double2 tile[N];
+
+	const uint32_t L = P.L, g = P.g;
+	// base: tile index spread over the free index bits
+	uint64_t base = (uint64_t)blockIdx.x << L;
+	for (uint32_t b = 0; b < g; b++)
+		base = insert_zero(base, P.piv[b]);
+
+	const uint32_t lmask = (1u << L) - 1;
+	const uint32_t tid = threadIdx.x;
+
+	// load: element e = tid + m*256; row = e >> L, col = e & lmask
+#pragma unroll 8
+	for (uint32_t m = 0; m < PER; m++) {
+		const uint32_t e = tid + m * kBlock;
+		const uint32_t row = e >> L;
+		uint64_t off = base | (e & lmask);
+		for (uint32_t b = 0; b < g; b++)
+			if (row >> b & 1)
+				off ^= P.w[b];
+		tile[e] = a[off];
+	}
+	__syncthreads();
+
+	bool prev_diag = false;
+	for (uint32_t k = 0; k < P.nterms; k++) {
+		const FusedTerm T = P.t[k];
+		const uint32_t bp = __popcll(base & T.z) & 1;
+		if (T.xt == 0) {
+			if (k > 0 && !prev_diag)
+				__syncthreads();
+#pragma unroll 4
+			for (uint32_t m = 0; m < PER; m++) {
+				const uint32_t e = tid + m * kBlock;
+				const int par = (__popc(e & T.zt) & 1) ^ bp;
+				rot_diag(tile[e], par, T.c, T.s);
+			}
+			prev_diag = true;
+		} else {
+			if (k > 0)
+				__syncthreads();
+#pragma unroll 4
+			for (uint32_t m = 0; m < PER / 2; m++) {
+				const uint32_t p = tid + m * kBlock;
+				const uint32_t vi = insert_zero32(p, T.h);
+				const uint32_t vj = vi ^ T.xt;
+				double2 xi = tile[vi], xj = tile[vj];
+				const int par = (__popc(vi & T.zt) & 1) ^ bp;
+				rot_pair(xi, xj, (int)((T.y + 2 * par) & 3), T.c,
+					T.s);
+				tile[vi] = xi;
+				tile[vj] = xj;
+			}
+			prev_diag = false;
+		}
+	}
+	__syncthreads();
+
+#pragma unroll 8
+	for (uint32_t m = 0; m < PER; m++) {
+		const uint32_t e = tid + m * kBlock;
+		const uint32_t row = e >> L;
+		uint64_t off = base | (e & lmask);
+		for (uint32_t b = 0; b < g; b++)
+			if (row >> b & 1)
+				off ^= P.w[b];
+		a[off] = tile[e];
+	}
+}
+
+// ---------------------------------------------------------------------------
+// small kernels
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_fill(double2 *__restrict__ a,
+	uint64_t n)
+{
+	const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+	for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+		i += stride)
+		a[i] = make_double2(0.0, 0.0);
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather(
+	const double2 *__restrict__ a, const uint64_t *__restrict__ idx,
+	double2 *__restrict__ out, uint64_t n)
+{
+	const uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+	if (k < n)
+		out[k] = a[idx[k]];
+}
+
+__device__ __forceinline__ uint64_t taper_idx(uint64_t v, uint32_t n_sites)
+{
+	const uint64_t lo = (v >> 1) & ((UINT64_C(1) << (n_sites - 1)) - 1);
+	return lo | ((v >> (n_sites + 1)) << (n_sites - 1));
+}
+
+__global__ __launch_bounds__(kBlock) void k_sc_scatter(
+	double2 *__restrict__ a, const uint64_t *__restrict__ ma,
+	const double *__restrict__ da, const uint64_t *__restrict__ mb,
+	const double *__restrict__ db, uint64_t Ma, uint64_t Mb,
+	uint32_t n_sites, double weight, int tapered, int accumulate)
+{
+	const uint64_t total = Ma * Mb;
+	const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+	for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+		k < total; k += stride) {
+		const uint64_t ia = k / Mb, ib = k - ia * Mb;
+		const double cf = mul(mul(weight, da[ia]), db[ib]);
+		if (fabs(cf) < 1.0e-12)
+			continue;
+		uint64_t idx = ma[ia] | (mb[ib] << n_sites);
+		if (tapered)
+			idx = taper_idx(idx, n_sites);
+		if (accumulate)
+			a[idx].x = add(a[idx].x, cf);
+		else
+			a[idx] = make_double2(cf, 0.0);
+	}
+}
+
+// Wavefront (64 lanes) sum via cross-lane shuffles, then the 4 waves of the
+// block through LDS; thread 0 returns the block total.
+__device__ __forceinline__ double2 block_sum2(double2 v)
+{
+	__shared__ double2 part[kBlock / 64];
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1) {
+		v.x = add(v.x, __shfl_xor(v.x, off, 64));
+		v.y = add(v.y, __shfl_xor(v.y, off, 64));
+	}
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	if (lane == 0)
+		part[wave] = v;
+	__syncthreads();
+	double2 r = make_double2(0.0, 0.0);
+	if (threadIdx.x == 0)
+		for (int w = 0; w < kBlock / 64; w++)
+			r = make_double2(add(r.x, part[w].x), add(r.y, part[w].y));
+	return r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sc_inner(
+	const double2 *__restrict__ a, const uint64_t *__restrict__ ma,
+	const double *__restrict__ da, const uint64_t *__restrict__ mb,
+	const double *__restrict__ db, uint64_t Ma, uint64_t Mb,
+	uint32_t n_sites, double weight, int tapered,
+	double2 *__restrict__ partial)
+{
+	const uint64_t total = Ma * Mb;
+	const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+	double2 acc = make_double2(0.0, 0.0);
+	for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+		k < total; k += stride) {
+		const uint64_t ia = k / Mb, ib = k - ia * Mb;
+		const double cf = mul(mul(weight, da[ia]), db[ib]);
+		if (fabs(cf) < 1.0e-12)
+			continue;
+		uint64_t idx = ma[ia] | (mb[ib] << n_sites);
+		if (tapered)
+			idx = taper_idx(idx, n_sites);
+		const double2 v = a[idx];
+		acc.x = add(acc.x, mul(cf, v.x));
+		acc.y = add(acc.y, mul(cf, v.y));
+	}
+	const double2 r = block_sum2(acc);
+	if (threadIdx.x == 0)
+		partial[blockIdx.x] = r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_norm2(
+	const double2 *__restrict__ a, uint64_t n, double2 *__restrict__ partial)
+{
+	const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+	double2 acc = make_double2(0.0, 0.0);
+	for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+		i += stride) {
+		const double2 v = a[i];
+		acc.x = add(acc.x, add(mul(v.x, v.x), mul(v.y, v.y)));
+	}
+	const double2 r = block_sum2(acc);
+	if (threadIdx.x == 0)
+		partial[blockIdx.x] = r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_reduce2(
+	const double2 *__restrict__ partial, uint32_t n, double2 *__restrict__ out)
+{
+	double2 acc = make_double2(0.0, 0.0);
+	for (uint32_t i = threadIdx.x; i < n; i += kBlock)
+		acc = make_double2(add(acc.x, partial[i].x),
+			add(acc.y, partial[i].y));
+	const double2 r = block_sum2(acc);
+	if (threadIdx.x == 0)
+		*out = r;
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+
+constexpr uint32_t kRedBlocks = 1024;
+
+// -- per-launch event timing ------------------------------------------------
+
+constexpr int kProfPool = 4096;
+bool g_prof = false;
+hipEvent_t g_pev[kProfPool][2];
+int g_pkind[kProfPool];
+uint32_t g_pterms[kProfPool];
+int g_pn = 0, g_pev_made = 0;
+ph2hip_prof g_pacc = {};
+hipEvent_t g_marks[8];
+bool g_marks_made = false;
+
+int prof_drain()
+{
+	for (int i = 0; i < g_pn; i++) {
+		float ms = 0.f;
+		HIPCHK(hipEventSynchronize(g_pev[i][1]));
+		HIPCHK(hipEventElapsedTime(&ms, g_pev[i][0], g_pev[i][1]));
+		g_pacc.launches[g_pkind[i]]++;
+		g_pacc.ms[g_pkind[i]] += ms;
+		g_pacc.terms[g_pkind[i]] += g_pterms[i];
+	}
+	g_pn = 0;
+	return 0;
+}
+
+// Event pair for the next launch (nullptr events when profiling is off).
+int prof_begin(int kind, uint32_t nterms, hipEvent_t *stop)
+{
+	*stop = nullptr;
+	if (!g_prof)
+		return 0;
+	if (g_pn == kProfPool && prof_drain())
+		return -1;
+	if (g_pn == g_pev_made) {
+		HIPCHK(hipEventCreate(&g_pev[g_pn][0]));
+		HIPCHK(hipEventCreate(&g_pev[g_pn][1]));
+		g_pev_made++;
+	}
+	g_pkind[g_pn] = kind;
+	g_pterms[g_pn] = nterms;
+	HIPCHK(hipEventRecord(g_pev[g_pn][0], g_stream));
+	*stop = g_pev[g_pn][1];
+	g_pn++;
+	return 0;
+}
+
+int prof_end(hipEvent_t stop)
+{
+	if (stop)
+		HIPCHK(hipEventRecord(stop, g_stream));
+	return 0;
+}
+
+int ensure_red()
+{
+	if (g_red)
+		return 0;
+	HIPCHK(hipMalloc(&g_red, sizeof(double2) * (kRedBlocks + 1)));
+	g_red_cap = kRedBlocks + 1;
+	return 0;
+}
+
+uint32_t grid_for(uint64_t n, uint32_t per_block, uint32_t cap)
+{
+	uint64_t g = (n + per_block - 1) / per_block;
+	if (g == 0)
+		g = 1;
+	return (uint32_t)(g < cap ? g : cap);
+}
+
+int top_bit(uint64_t v)
+{
+	return 63 - __builtin_clzll(v);
+}
+
+// Reduced basis of span{x & ~lowmask}: vectors with distinct leading bits,
+// each zero at every other leading bit; returns the rank.
+uint32_t reduce_basis(const struct ph2hip_rot *rot, size_t n, uint64_t lowmask,
+	uint64_t *w, uint32_t cap)
+{
+	uint32_t g = 0;
+	for (size_t k = 0; k < n; k++) {
+		uint64_t v = rot[k].x & ~lowmask;
+		for (uint32_t b = 0; b < g && v; b++)
+			if (v >> top_bit(w[b]) & 1)
+				v ^= w[b];
+		if (!v)
+			continue;
+		if (g == cap)
+			return cap + 1;
+		// clear the new leading bit from the others, insert
+		const int p = top_bit(v);
+		for (uint32_t b = 0; b < g; b++)
+			if (w[b] >> p & 1)
+				w[b] ^= v;
+		w[g++] = v;
+	}
+	// sort by leading bit, ascending
+	for (uint32_t i = 1; i < g; i++)
+		for (uint32_t j = i; j > 0 && top_bit(w[j - 1]) > top_bit(w[j]); j--) {
+			const uint64_t t = w[j];
+			w[j] = w[j - 1];
+			w[j - 1] = t;
+		}
+	return g;
+}
+
+template <int TB>
+int launch_fused(double2 *a, uint32_t nqb, const FusedPass &P)
+{
+	const uint64_t ntiles = UINT64_C(1) << (nqb - TB);
+	hipEvent_t stop;
+	if (prof_begin(2, P.nterms, &stop))
+		return -1;
+	hipLaunchKernelGGL(k_rot_fused<TB>, dim3((uint32_t)ntiles),
+		dim3(kBlock), 0, g_stream, a, P);
+	HIPCHK(hipGetLastError());
+	return prof_end(stop);
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+int ph2hip_init(int device)
+{
+	if (g_stream)
+		return 0;
+	if (device < 0) {
+		const char *env = getenv("PHASE2_HIP_DEVICE");
+		device = env ? atoi(env) : 0;
+	}
+	int ndev = 0;
+	HIPCHK(hipGetDeviceCount(&ndev));
+	if (device >= ndev)
+		return fail("device %d not present (%d devices)", device, ndev);
+	HIPCHK(hipSetDevice(device));
+	HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+	g_device = device;
+	return 0;
+}
+
+void ph2hip_shutdown(void)
+{
+	if (!g_stream)
+		return;
+	(void)hipStreamSynchronize(g_stream);
+	if (g_red)
+		(void)hipFree(g_red);
+	g_red = nullptr;
+	(void)hipStreamDestroy(g_stream);
+	g_stream = nullptr;
+}
+
+const char *ph2hip_strerror(void)
+{
+	return g_err;
+}
+
+int ph2hip_synchronize(void)
+{
+	if (ensure_init())
+		return -1;
+	HIPCHK(hipStreamSynchronize(g_stream));
+	return 0;
+}
+
+int ph2hip_malloc(void **dptr, size_t bytes)
+{
+	if (ensure_init())
+		return -1;
+	*dptr = nullptr;
+	HIPCHK(hipMalloc(dptr, bytes ? bytes : 16));
+	return 0;
+}
+
+int ph2hip_free(void *dptr)
+{
+	if (!dptr)
+		return 0;
+	if (g_stream)
+		HIPCHK(hipStreamSynchronize(g_stream));
+	HIPCHK(hipFree(dptr));
+	return 0;
+}
+
+int ph2hip_h2d(void *dst, const void *src, size_t bytes)
+{
+	if (ensure_init())
+		return -1;
+	HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream));
+	HIPCHK(hipStreamSynchronize(g_stream));
+	return 0;
+}
+
+int ph2hip_d2h(void *dst, const void *src, size_t bytes)
+{
+	if (ensure_init())
+		return -1;
+	HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
+	HIPCHK(hipStreamSynchronize(g_stream));
+	return 0;
+}
+
+int ph2hip_d2d(void *dst, const void *src, size_t bytes)
+{
+	if (ensure_init())
+		return -1;
+	HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice,
+		g_stream));
+	return 0;
+}
+
+int ph2hip_zero(void *amp, uint64_t namp)
+{
+	if (ensure_init())
+		return -1;
+	hipLaunchKernelGGL(k_fill, dim3(grid_for(namp, kBlock * 4, 1u << 20)),
+		dim3(kBlock), 0, g_stream, (double2 *)amp, namp);
+	HIPCHK(hipGetLastError());
+	return 0;
+}
+
+int ph2hip_set1(void *amp, uint64_t i, double re, double im)
+{
+	const double v[2] = { re, im };
+	return ph2hip_h2d((double2 *)amp + i, v, sizeof v);
+}
+
+int ph2hip_get1(const void *amp, uint64_t i, double *re_im)
+{
+	return ph2hip_d2h(re_im, (const double2 *)amp + i, 2 * sizeof(double));
+}
+
+int ph2hip_rotate_plain(void *amp, uint32_t nqb, const struct ph2hip_rot *rot,
+	size_t n)
+{
+	if (ensure_init())
+		return -1;
+	if (nqb == 0 || nqb > 40)
+		return fail("rotate_plain: nqb=%u out of range", nqb);
+	const uint64_t namp = UINT64_C(1) << nqb;
+	double2 *a = (double2 *)amp;
+	constexpr int UNR = 2;
+	for (size_t k = 0; k < n; k++) {
+		const uint64_t x = rot[k].x, z = rot[k].z;
+		if (x >> nqb)
+			return fail("rotate_plain: term %zu flips qubit >= %u", k,
+				nqb);
+		hipEvent_t stop;
+		if (prof_begin(x == 0 ? 1 : 0, 1, &stop))
+			return -1;
+		if (x == 0) {
+			hipLaunchKernelGGL(k_rot_diag<UNR>,
+				dim3(grid_for(namp, kBlock * UNR, 0x7fffffff)),
+				dim3(kBlock), 0, g_stream, a, namp, z, rot[k].c,
+				rot[k].s);
+		} else {
+			const uint64_t npairs = namp / 2;
+			const uint32_t y = __builtin_popcountll(x & z) & 3;
+			hipLaunchKernelGGL(k_rot_pair<UNR>,
+				dim3(grid_for(npairs, kBlock * UNR, 0x7fffffff)),
+				dim3(kBlock), 0, g_stream, a, npairs, x, z,
+				(uint32_t)top_bit(x), y, rot[k].c, rot[k].s);
+		}
+		HIPCHK(hipGetLastError());
+		if (prof_end(stop))
+			return -1;
+	}
+	return 0;
+}
+
+uint32_t ph2hip_span_rank(const struct ph2hip_rot *rot, size_t n,
+	uint32_t low_bits)
+{
+	uint64_t w[64];
+	const uint64_t lowmask = low_bits >= 64 ? ~UINT64_C(0)
+		: (UINT64_C(1) << low_bits) - 1;
+	return reduce_basis(rot, n, lowmask, w, 64);
+}
+
+int ph2hip_rotate_fused(void *amp, uint32_t nqb, const struct ph2hip_rot *rot,
+	size_t n, const uint32_t *pass_len, const uint32_t *tile_bits,
+	size_t npass, uint32_t low_bits)
+{
+	if (ensure_init())
+		return -1;
+	if (nqb == 0 || nqb > 40)
+		return fail("rotate_fused: nqb=%u out of range", nqb);
+	size_t tot = 0;
+	for (size_t p = 0; p < npass; p++) {
+		if (pass_len[p] == 0 || pass_len[p] > PH2HIP_PASS_MAX)
+			return fail("rotate_fused: pass %zu has %u terms", p,
+				pass_len[p]);
+		const uint32_t tb = tile_bits[p];
+		if (tb < PH2HIP_TILE_MIN || tb > PH2HIP_TILE_MAX || tb > nqb)
+			return fail("rotate_fused: tile_bits %u invalid for "
+				    "nqb=%u", tb, nqb);
+		tot += pass_len[p];
+	}
+	if (tot != n)
+		return fail("rotate_fused: passes cover %zu of %zu terms", tot, n);
+	for (size_t k = 0; k < n; k++)
+		if (rot[k].x >> nqb)
+			return fail("rotate_fused: term %zu flips qubit >= %u", k,
+				nqb);
+
+	// Build every pass first, so nothing is applied if one does not fit.
+	FusedPass *passes = (FusedPass *)calloc(npass ? npass : 1,
+		sizeof(FusedPass));
+	if (!passes)
+		return fail("rotate_fused: out of host memory");
+	size_t k0 = 0;
+	for (size_t p = 0; p < npass; p++) {
+		FusedPass &P = passes[p];
+		const uint32_t tb = tile_bits[p];
+		const struct ph2hip_rot *r = rot + k0;
+		const size_t len = pass_len[p];
+		// Grow the contiguous low part L while the gathered rank g fits:
+		// L = tb - g(L) at the fixed point (g is non-increasing in L).
+		uint32_t L = low_bits < tb ? low_bits : tb;
+		uint32_t g = 0;
+		uint64_t w[PH2HIP_TILE_MAX + 1];
+		for (;;) {
+			const uint64_t lm = (UINT64_C(1) << L) - 1;
+			g = reduce_basis(r, len, lm, w, tb);
+			if (L + g > tb) {
+				free(passes);
+				return fail("rotate_fused: pass %zu needs %u+%u > %u "
+					    "tile bits", p, L, g, tb);
+			}
+			if (L + g == tb)
+				break;
+			L = tb - g;
+		}
+		P.L = L;
+		P.g = g;
+		P.nterms = (uint32_t)len;
+		for (uint32_t b = 0; b < g; b++) {
+			P.w[b] = w[b];
+			P.piv[b] = (uint32_t)top_bit(w[b]);
+		}
+		const uint64_t lm = (UINT64_C(1) << L) - 1;
+		for (size_t k = 0; k < len; k++) {
+			const uint64_t x = r[k].x, z = r[k].z;
+			uint32_t xt = (uint32_t)(x & lm), zt = (uint32_t)(z & lm);
+			uint64_t rest = x & ~lm;
+			for (uint32_t b = 0; b < g; b++) {
+				if (x >> P.piv[b] & 1) {
+					xt |= 1u << (L + b);
+					rest ^= w[b];
+				}
+				if (__builtin_popcountll(w[b] & z) & 1)
+					zt |= 1u << (L + b);
+			}
+			if (rest) {
+				free(passes);
+				return fail("rotate_fused: term %zu outside its tile",
+					k0 + k);
+			}
+			FusedTerm &T = P.t[k];
+			T.z = z;
+			T.xt = xt;
+			T.zt = zt;
+			T.c = r[k].c;
+			T.s = r[k].s;
+			T.y = __builtin_popcountll(x & z) & 3;
+			T.h = xt ? (uint32_t)(31 - __builtin_clz(xt)) : 0;
+		}
+		k0 += len;
+	}
+	double2 *a = (double2 *)amp;
+	int rc = 0;
+	for (size_t p = 0; p < npass && rc == 0; p++) {
+		switch (tile_bits[p]) {
+		case 9: rc = launch_fused<9>(a, nqb, passes[p]); break;
+		case 10: rc = launch_fused<10>(a, nqb, passes[p]); break;
+		case 11: rc = launch_fused<11>(a, nqb, passes[p]); break;
+		case 12: rc = launch_fused<12>(a, nqb, passes[p]); break;
+		case 13: rc = launch_fused<13>(a, nqb, passes[p]); break;
+		default: rc = fail("rotate_fused: tile_bits"); break;
+		}
+	}
+	free(passes);
+	return rc;
+}
+
+int ph2hip_gather(const void *amp, uint32_t nqb, const uint64_t *idx,
+	size_t n, double *out)
+{
+	if (ensure_init())
+		return -1;
+	if (n == 0)
+		return 0;
+	const uint64_t namp = UINT64_C(1) << nqb;
+	for (size_t k = 0; k < n; k++)
+		if (idx[k] >= namp)
+			return fail("gather: index %llu beyond 2^%u",
+				(unsigned long long)idx[k], nqb);
+	uint64_t *didx = nullptr;
+	double2 *dout = nullptr;
+	HIPCHK(hipMalloc(&didx, sizeof(uint64_t) * n));
+	if (hipMalloc(&dout, sizeof(double2) * n) != hipSuccess) {
+		(void)hipFree(didx);
+		return fail("gather: hipMalloc");
+	}
+	int rc = 0;
+	if (hipMemcpyAsync(didx, idx, sizeof(uint64_t) * n,
+		    hipMemcpyHostToDevice, g_stream) != hipSuccess)
+		rc = fail("gather: copy in");
+	if (!rc) {
+		hipLaunchKernelGGL(k_gather, dim3(grid_for(n, kBlock, 1u << 30)),
+			dim3(kBlock), 0, g_stream, (const double2 *)amp, didx,
+			dout, (uint64_t)n);
+		if (hipGetLastError() != hipSuccess)
+			rc = fail("gather: launch");
+	}
+	if (!rc && hipMemcpyAsync(out, dout, sizeof(double2) * n,
+			   hipMemcpyDeviceToHost, g_stream) != hipSuccess)
+		rc = fail("gather: copy out");
+	if (hipStreamSynchronize(g_stream) != hipSuccess && !rc)
+		rc = fail("gather: sync");
+	(void)hipFree(didx);
+	(void)hipFree(dout);
+	return rc;
+}
+
+struct ph2hip_sc {
+	uint64_t *ma, *mb;
+	double *da, *db;
+	size_t Ma, Mb;
+	uint64_t or_a, or_b;  // OR of all masks (index-range check)
+};
+
+int ph2hip_sc_create(struct ph2hip_sc **out, const uint64_t *mask_a,
+	size_t Ma, const uint64_t *mask_b, size_t Mb)
+{
+	if (ensure_init())
+		return -1;
+	*out = nullptr;
+	struct ph2hip_sc *sc = (struct ph2hip_sc *)calloc(1, sizeof *sc);
+	if (!sc)
+		return fail("sc_create: out of host memory");
+	sc->Ma = Ma;
+	sc->Mb = Mb;
+	for (size_t i = 0; i < Ma; i++)
+		sc->or_a |= mask_a[i];
+	for (size_t i = 0; i < Mb; i++)
+		sc->or_b |= mask_b[i];
+	if (hipMalloc(&sc->ma, sizeof(uint64_t) * (Ma ? Ma : 1)) != hipSuccess ||
+	    hipMalloc(&sc->mb, sizeof(uint64_t) * (Mb ? Mb : 1)) != hipSuccess ||
+	    hipMalloc(&sc->da, sizeof(double) * (Ma ? Ma : 1)) != hipSuccess ||
+	    hipMalloc(&sc->db, sizeof(double) * (Mb ? Mb : 1)) != hipSuccess) {
+		ph2hip_sc_destroy(sc);
+		return fail("sc_create: hipMalloc");
+	}
+	if (hipMemcpy(sc->ma, mask_a, sizeof(uint64_t) * Ma,
+		    hipMemcpyHostToDevice) != hipSuccess ||
+	    hipMemcpy(sc->mb, mask_b, sizeof(uint64_t) * Mb,
+		    hipMemcpyHostToDevice) != hipSuccess) {
+		ph2hip_sc_destroy(sc);
+		return fail("sc_create: copy");
+	}
+	*out = sc;
+	return 0;
+}
+
+int ph2hip_sc_set_dets(struct ph2hip_sc *sc, const double *det_a,
+	const double *det_b)
+{
+	if (ensure_init())
+		return -1;
+	HIPCHK(hipMemcpyAsync(sc->da, det_a, sizeof(double) * sc->Ma,
+		hipMemcpyHostToDevice, g_stream));
+	HIPCHK(hipMemcpyAsync(sc->db, det_b, sizeof(double) * sc->Mb,
+		hipMemcpyHostToDevice, g_stream));
+	HIPCHK(hipStreamSynchronize(g_stream));
+	return 0;
+}
+
+void ph2hip_sc_destroy(struct ph2hip_sc *sc)
+{
+	if (!sc)
+		return;
+	if (g_stream)
+		(void)hipStreamSynchronize(g_stream);
+	(void)hipFree(sc->ma);
+	(void)hipFree(sc->mb);
+	(void)hipFree(sc->da);
+	(void)hipFree(sc->db);
+	free(sc);
+}
+
+static int sc_check(const struct ph2hip_sc *sc, uint32_t nqb,
+	uint32_t n_sites, int tapered)
+{
+	if (n_sites == 0 || n_sites > 32)
+		return fail("sc: n_sites=%u out of range", n_sites);
+	if (sc->or_b && (2 * n_sites > 64))
+		return fail("sc: beta bits beyond 64");
+	uint64_t bound = sc->or_a | (sc->or_b << n_sites);
+	if (tapered) {
+		const uint64_t lo = (bound >> 1) &
+			((UINT64_C(1) << (n_sites - 1)) - 1);
+		bound = lo | ((bound >> (n_sites + 1)) << (n_sites - 1));
+	}
+	if (nqb < 64 && (bound >> nqb))
+		return fail("sc: determinant index beyond the %u-qubit register "
+			    "(n_sites=%u)", nqb, n_sites);
+	return 0;
+}
+
+int ph2hip_sc_scatter(void *amp, uint32_t nqb, const struct ph2hip_sc *sc,
+	uint32_t n_sites, double weight, int tapered, int accumulate)
+{
+	if (ensure_init() || sc_check(sc, nqb, n_sites, tapered))
+		return -1;
+	const uint64_t total = (uint64_t)sc->Ma * sc->Mb;
+	if (total == 0)
+		return 0;
+	hipLaunchKernelGGL(k_sc_scatter, dim3(grid_for(total, kBlock, 1u << 16)),
+		dim3(kBlock), 0, g_stream, (double2 *)amp, sc->ma, sc->da,
+		sc->mb, sc->db, (uint64_t)sc->Ma, (uint64_t)sc->Mb, n_sites,
+		weight, tapered, accumulate);
+	HIPCHK(hipGetLastError());
+	return 0;
+}
+
+int ph2hip_sc_inner(const void *amp, uint32_t nqb, const struct ph2hip_sc *sc,
+	uint32_t n_sites, double weight, int tapered, double *out)
+{
+	if (ensure_init() || sc_check(sc, nqb, n_sites, tapered) || ensure_red())
+		return -1;
+	const uint64_t total = (uint64_t)sc->Ma * sc->Mb;
+	const uint32_t nb = grid_for(total, kBlock, kRedBlocks);
+	double2 *partial = (double2 *)g_red;
+	hipLaunchKernelGGL(k_sc_inner, dim3(nb), dim3(kBlock), 0, g_stream,
+		(const double2 *)amp, sc->ma, sc->da, sc->mb, sc->db,
+		(uint64_t)sc->Ma, (uint64_t)sc->Mb, n_sites, weight, tapered,
+		partial);
+	HIPCHK(hipGetLastError());
+	hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(kBlock), 0, g_stream,
+		partial, nb, partial + kRedBlocks);
+	HIPCHK(hipGetLastError());
+	return ph2hip_d2h(out, partial + kRedBlocks, sizeof(double2));
+}
+
+int ph2hip_prof_enable(int on)
+{
+	if (ensure_init())
+		return -1;
+	if (!on && g_pn && prof_drain())
+		return -1;
+	g_prof = on != 0;
+	return 0;
+}
+
+int ph2hip_prof_read(struct ph2hip_prof *out, int reset)
+{
+	if (ensure_init() || prof_drain())
+		return -1;
+	*out = g_pacc;
+	if (reset)
+		g_pacc = ph2hip_prof{};
+	return 0;
+}
+
+int ph2hip_mark(int slot)
+{
+	if (ensure_init())
+		return -1;
+	if (slot < 0 || slot >= 8)
+		return fail("mark: slot %d", slot);
+	if (!g_marks_made) {
+		for (auto &e : g_marks)
+			HIPCHK(hipEventCreate(&e));
+		g_marks_made = true;
+	}
+	HIPCHK(hipEventRecord(g_marks[slot], g_stream));
+	return 0;
+}
+
+int ph2hip_elapsed_ms(int a, int b, double *ms)
+{
+	if (!g_marks_made || a < 0 || a >= 8 || b < 0 || b >= 8)
+		return fail("elapsed: no marks");
+	float f = 0.f;
+	HIPCHK(hipEventSynchronize(g_marks[b]));
+	HIPCHK(hipEventElapsedTime(&f, g_marks[a], g_marks[b]));
+	*ms = f;
+	return 0;
+}
+
+int ph2hip_norm2(const void *amp, uint32_t nqb, double *out)
+{
+	if (ensure_init() || ensure_red())
+		return -1;
+	const uint64_t namp = UINT64_C(1) << nqb;
+	const uint32_t nb = grid_for(namp, kBlock * 8, kRedBlocks);
+	double2 *partial = (double2 *)g_red;
+	hipLaunchKernelGGL(k_norm2, dim3(nb), dim3(kBlock), 0, g_stream,
+		(const double2 *)amp, namp, partial);
+	HIPCHK(hipGetLastError());
+	hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(kBlock), 0, g_stream,
+		partial, nb, partial + kRedBlocks);
+	HIPCHK(hipGetLastError());
+	double r[2];
+	if (ph2hip_d2h(r, partial + kRedBlocks, sizeof r))
+		return -1;
+	*out = r[0];
+	return 0;
+}
+
+} // extern "C"

@@ -1,0 +1,270 @@
+/*
+ * BACKEND=hip register (replaces phase2/qreg.c, phase2/qreg_qreg.c and
+ * phase2/qreg_cuda.c of the reference on one GPU).
+ *
+ * One hipMalloc buffer holds the whole register; with one process there is
+ * no hi/lo split (qb_hi = 0), no partner exchange and no mix/add passes:
+ * the reference's single-rank flush (self-exchange, mix with bm = 1, rotate,
+ * add 0) reduces exactly to in-order rotations, which is what
+ * qreg_paulirot applies here.  cos/sin come from the host libm per term as
+ * in phase2/qreg_qreg.c:189.
+ *
+ * The void entry points cannot return errors; a failed device call logs
+ * and sets a sticky flag read by qreg_error().
+ */
+#define LOG_SUBSYS "qreg"
+
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "log.h"
+#include "phase2/qreg.h"
+#include "phase2/qreg_hip.h"
+#include "phase2/world.h"
+#include "phase2_hip.h"
+
+struct qreg_hip {
+	void *damp;
+	int err;
+	int mode;
+	uint32_t tile_bits, low_bits;
+	struct ph2hip_rot *rot;
+	size_t rot_cap;
+	struct ph2_sched sched;
+};
+
+static int g_mode = -1;
+static uint32_t g_tile = 12, g_low = 6;
+static ph2_sched_fn g_sched_fn;
+static void *g_sched_ctx;
+
+static void load_defaults(void)
+{
+	if (g_mode >= 0)
+		return;
+	g_mode = QREG_HIP_FUSED;
+	const char *m = getenv("PHASE2_HIP_MODE");
+	if (m && strcmp(m, "plain") == 0)
+		g_mode = QREG_HIP_PLAIN;
+	const char *t = getenv("PHASE2_HIP_TILE");
+	if (t) {
+		const long v = strtol(t, NULL, 10);
+		if (v >= PH2HIP_TILE_MIN && v <= PH2HIP_TILE_MAX)
+			g_tile = (uint32_t)v;
+	}
+	const char *l = getenv("PHASE2_HIP_LOW");
+	if (l) {
+		const long v = strtol(l, NULL, 10);
+		if (v >= 0 && v <= PH2HIP_TILE_MAX)
+			g_low = (uint32_t)v;
+	}
+}
+
+void qreg_hip_set_default(int mode, uint32_t tile_bits, uint32_t low_bits)
+{
+	load_defaults();
+	g_mode = mode;
+	g_tile = tile_bits;
+	g_low = low_bits;
+}
+
+void qreg_hip_configure(struct qreg *reg, int mode, uint32_t tile_bits,
+	uint32_t low_bits)
+{
+	struct qreg_hip *h = reg->backend;
+	if (!h)
+		return;
+	h->mode = mode;
+	h->tile_bits = tile_bits;
+	h->low_bits = low_bits;
+}
+
+void ph2_sched_install(ph2_sched_fn fn, void *ctx)
+{
+	g_sched_fn = fn;
+	g_sched_ctx = ctx;
+}
+
+void *qreg_hip_device_ptr(struct qreg *reg)
+{
+	struct qreg_hip *h = reg->backend;
+	return h ? h->damp : NULL;
+}
+
+static void set_err(struct qreg *reg, const char *what)
+{
+	struct qreg_hip *h = reg->backend;
+	if (h)
+		h->err = -1;
+	log_error("%s: %s", what, ph2hip_strerror());
+}
+
+int qreg_error(const struct qreg *reg)
+{
+	const struct qreg_hip *h = reg->backend;
+	return h ? h->err : -1;
+}
+
+void qreg_clear_error(struct qreg *reg)
+{
+	struct qreg_hip *h = reg->backend;
+	if (h)
+		h->err = 0;
+}
+
+int qreg_init(struct qreg *reg, uint32_t qb)
+{
+	memset(reg, 0, sizeof *reg);
+	if (world_info(&reg->wd) != WORLD_READY) {
+		log_error("qreg_init: world not ready");
+		return -1;
+	}
+	if (qb == 0 || qb > 40) {
+		log_error("qreg_init: nqb=%u outside 1..40", qb);
+		return -1;
+	}
+	load_defaults();
+	struct qreg_hip *h = calloc(1, sizeof *h);
+	if (!h) {
+		log_error("qreg_init: alloc backend");
+		return -1;
+	}
+	const uint64_t namp = UINT64_C(1) << qb;
+	if (ph2hip_malloc(&h->damp, namp * 16) < 0) {
+		log_error("qreg_init: device alloc of %llu bytes: %s",
+			(unsigned long long)(namp * 16), ph2hip_strerror());
+		free(h);
+		return -1;
+	}
+	if (qb <= QREG_SHADOW_MAX_QB) {
+		reg->amp = malloc(sizeof(_Complex double) * namp);
+		if (!reg->amp) {
+			log_error("qreg_init: host shadow alloc");
+			ph2hip_free(h->damp);
+			free(h);
+			return -1;
+		}
+	}
+	h->mode = g_mode;
+	h->tile_bits = g_tile;
+	h->low_bits = g_low;
+	reg->qb_lo = qb;
+	reg->qb_hi = 0;
+	reg->buf = NULL;
+	reg->namp = namp;
+	reg->backend = h;
+	return 0;
+}
+
+void qreg_free(struct qreg *reg)
+{
+	struct qreg_hip *h = reg->backend;
+	if (h) {
+		ph2hip_free(h->damp);
+		free(h->rot);
+		ph2_sched_free(&h->sched);
+		free(h);
+	}
+	free(reg->amp);
+	reg->amp = NULL;
+	reg->backend = NULL;
+}
+
+void qreg_getamp(struct qreg *reg, uint64_t i, _Complex double *z)
+{
+	struct qreg_hip *h = reg->backend;
+	double v[2] = { 0.0, 0.0 };
+	if (ph2hip_get1(h->damp, i & (reg->namp - 1), v) < 0)
+		set_err(reg, "qreg_getamp");
+	*z = CMPLX(v[0], v[1]);
+}
+
+void qreg_setamp(struct qreg *reg, uint64_t i, _Complex double z)
+{
+	struct qreg_hip *h = reg->backend;
+	if (ph2hip_set1(h->damp, i & (reg->namp - 1), creal(z), cimag(z)) < 0)
+		set_err(reg, "qreg_setamp");
+}
+
+void qreg_zero(struct qreg *reg)
+{
+	struct qreg_hip *h = reg->backend;
+	if (ph2hip_zero(h->damp, reg->namp) < 0)
+		set_err(reg, "qreg_zero");
+}
+
+void qreg_sync_host_to_device(struct qreg *reg)
+{
+	struct qreg_hip *h = reg->backend;
+	if (!reg->amp) {
+		log_error("qreg_sync_host_to_device: no host shadow above %d "
+			  "qubits", QREG_SHADOW_MAX_QB);
+		h->err = -1;
+		return;
+	}
+	if (ph2hip_h2d(h->damp, reg->amp, reg->namp * 16) < 0)
+		set_err(reg, "qreg_sync_host_to_device");
+}
+
+void qreg_sync_device_to_host(struct qreg *reg)
+{
+	struct qreg_hip *h = reg->backend;
+	if (!reg->amp) {
+		log_error("qreg_sync_device_to_host: no host shadow above %d "
+			  "qubits", QREG_SHADOW_MAX_QB);
+		h->err = -1;
+		return;
+	}
+	if (ph2hip_d2h(reg->amp, h->damp, reg->namp * 16) < 0)
+		set_err(reg, "qreg_sync_device_to_host");
+}
+
+void qreg_paulirot(struct qreg *reg, struct paulis code_hi,
+	const struct paulis *codes_lo, const double *phis, size_t ncodes)
+{
+	struct qreg_hip *h = reg->backend;
+	if (code_hi.pak[0] | code_hi.pak[1]) {
+		log_error("qreg_paulirot: hi code on a single-GPU register");
+		h->err = -1;
+		return;
+	}
+	if (ncodes == 0)
+		return;
+	if (ncodes > h->rot_cap) {
+		struct ph2hip_rot *r = realloc(h->rot, sizeof *r * ncodes);
+		if (!r) {
+			log_error("qreg_paulirot: alloc");
+			h->err = -1;
+			return;
+		}
+		h->rot = r;
+		h->rot_cap = ncodes;
+	}
+	for (size_t k = 0; k < ncodes; k++) {
+		h->rot[k].x = codes_lo[k].pak[0];
+		h->rot[k].z = codes_lo[k].pak[1];
+		h->rot[k].c = cos(phis[k]);
+		h->rot[k].s = sin(phis[k]);
+	}
+	const uint32_t nqb = reg->qb_lo;
+	int rc;
+	if (h->mode == QREG_HIP_PLAIN || nqb < PH2HIP_TILE_MIN) {
+		rc = ph2hip_rotate_plain(h->damp, nqb, h->rot, ncodes);
+	} else {
+		rc = -1;
+		if (g_sched_fn)
+			rc = g_sched_fn(g_sched_ctx, h->rot, ncodes, nqb,
+				&h->sched);
+		if (rc < 0)
+			rc = ph2_sched_greedy(h->rot, ncodes, nqb, h->tile_bits,
+				h->low_bits, &h->sched);
+		if (rc == 0)
+			rc = ph2hip_rotate_fused(h->damp, nqb, h->rot, ncodes,
+				h->sched.len, h->sched.tile_bits, h->sched.npass,
+				h->sched.low_bits);
+	}
+	if (rc < 0)
+		set_err(reg, "qreg_paulirot");
+}

@@ -1,0 +1,85 @@
+/*
+ * Fusion schedules: how a block of consecutive rotations is cut into HBM
+ * passes (no counterpart in the reference; the block itself is the
+ * circ_cache batch of phase2/circ_cache.c:55-76).
+ */
+#include <stdint.h>
+#include <stdlib.h>
+
+#include "phase2/qreg_hip.h"
+
+void ph2_sched_free(struct ph2_sched *s)
+{
+	free(s->len);
+	free(s->tile_bits);
+	s->len = s->tile_bits = NULL;
+	s->npass = s->cap = 0;
+}
+
+static int sched_push(struct ph2_sched *s, uint32_t len, uint32_t tb)
+{
+	if (s->npass == s->cap) {
+		const size_t cap = s->cap ? 2 * s->cap : 64;
+		uint32_t *a = realloc(s->len, sizeof *a * cap);
+		if (!a)
+			return -1;
+		s->len = a;
+		uint32_t *b = realloc(s->tile_bits, sizeof *b * cap);
+		if (!b)
+			return -1;
+		s->tile_bits = b;
+		s->cap = cap;
+	}
+	s->len[s->npass] = len;
+	s->tile_bits[s->npass] = tb;
+	s->npass++;
+	return 0;
+}
+
+static inline int topbit(uint64_t v)
+{
+	return 63 - __builtin_clzll(v);
+}
+
+int ph2_sched_greedy(const struct ph2hip_rot *rot, size_t n, uint32_t nqb,
+	uint32_t tile_bits, uint32_t low_bits, struct ph2_sched *out)
+{
+	out->npass = 0;
+	if (tile_bits > nqb)
+		tile_bits = nqb;
+	if (low_bits > tile_bits)
+		low_bits = tile_bits;
+	out->low_bits = low_bits;
+	const uint64_t lowmask = (UINT64_C(1) << low_bits) - 1;
+	const uint32_t gmax = tile_bits - low_bits;
+	uint64_t basis[64];
+	int piv[64];
+	uint32_t g = 0, len = 0;
+	for (size_t k = 0; k < n; k++) {
+		uint64_t v = rot[k].x & ~lowmask;
+		for (uint32_t b = 0; b < g && v; b++)
+			if (v >> piv[b] & 1)
+				v ^= basis[b];
+		const int grows = v != 0;
+		if (len == PH2HIP_PASS_MAX || (grows && g == gmax)) {
+			if (sched_push(out, len, tile_bits) < 0)
+				return -1;
+			g = 0;
+			len = 0;
+			v = rot[k].x & ~lowmask;
+		}
+		if (v) {
+			const int p = topbit(v);
+			for (uint32_t b = 0; b < g; b++)
+				if (basis[b] >> p & 1)
+					basis[b] ^= v;
+			basis[g] = v;
+			piv[g] = p;
+			g++;
+		}
+		len++;
+	}
+	if (len && sched_push(out, len, tile_bits) < 0)
+		return -1;
+	return 0;
+}

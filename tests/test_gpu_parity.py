@@ -1,0 +1,273 @@
+"""HIP backend against the reference's golden vectors and the CPU oracle.
+
+Bit-exact (``array_equal`` on the float64 view; -0.0 == +0.0) for every
+rotation, sweep, multidet overlap and scatter index/value; the
+coefficient-matrix inner product reduces in a tree on the device, so its
+overlaps are compared at 1e-12 relative."""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from phase2_amd import _native as nat
+from phase2_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-12
+MODES = [(nat.PLAIN, 12, 6), (nat.FUSED, 12, 6), (nat.FUSED, 9, 0),
+         (nat.FUSED, 13, 8), (nat.FUSED, 10, 4)]
+
+
+def close(a, b, rel=REL):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.max(np.abs(a - b)) <= rel * max(1.0, np.max(np.abs(b)))
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    print("gpu: init", flush=True)
+    nat.init()
+    yield
+    nat.set_default(nat.FUSED, 12, 6)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_rotate_n10_golden(golden, mode):
+    g = golden("rotate_n10.npz")
+    out = nat.rotate(g["state_in"], g["xm"], g["zm"], g["phi"], *mode)
+    assert np.array_equal(out, g["state_out"])
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_rotate_n13_golden(golden, mode):
+    g = golden("rotate_n13.npz")
+    out = nat.rotate(g["state_in"], g["xm"], g["zm"], g["phi"], *mode)
+    assert po.state_hash(out) == str(g["hash"])
+
+
+@pytest.mark.parametrize("mode", MODES[:2])
+def test_trott_n16_golden(golden, mode):
+    g = golden("trott_n16.npz")
+    w = synth.trott_n16()
+    nat.set_default(*mode)
+    ov, st = nat.trott(1, w["hm"], dict(det_idx=w["det_idx"],
+                                        det_cf=w["det_cf"]),
+                       w["delta"], w["steps"])
+    assert np.array_equal(ov, g["overlaps"])
+    assert po.state_hash(st) == str(g["hash"])
+
+
+def test_trott2_coeff_golden(golden):
+    g = golden("trott2_coeff_n12.npz")
+    hm = synth.hamiltonian(12, 150, seed=3, decay=True)
+    prep = dict(C_alpha=g["Ca"], n_sites=12, n_alpha=3, n_beta=0,
+                closed_shell=1, tapered=0)
+    nat.set_default(nat.FUSED, 10, 4)
+    ov, st = nat.trott(2, hm, prep, 0.05, 6)
+    assert close(ov, g["overlaps"])
+    assert po.state_hash(st) == str(g["hash"])
+
+
+def test_trott_open_tapered_golden(golden):
+    g = golden("trott_open_n12.npz")
+    hm = synth.hamiltonian(12, 120, seed=4)
+    nat.set_default(nat.FUSED, 11, 5)
+    ov, st = nat.trott(1, hm, dict(C_alpha=g["Ca"], C_beta=g["Cb"],
+                                   n_sites=6, n_alpha=2, n_beta=3,
+                                   closed_shell=0, tapered=0), 0.1, 5)
+    assert close(ov, g["overlaps"])
+    assert po.state_hash(st) == str(g["hash"])
+    ov, st = nat.trott(1, hm, dict(C_alpha=g["Ca7"], C_beta=g["Cb7"],
+                                   n_sites=7, n_alpha=2, n_beta=2,
+                                   closed_shell=0, tapered=1), 0.1, 5)
+    assert close(ov, g["overlaps_tap"])
+    assert po.state_hash(st) == str(g["hash_tap"])
+
+
+def test_qdrift_golden(golden):
+    g = golden("qdrift_n10.npz")
+    hm = synth.hamiltonian(10, 120, seed=5)
+    nat.set_default(nat.FUSED, 10, 3)
+    ov, st = nat.qdrift(hm, dict(det_idx=g["didx"], det_cf=g["dcf"]), 4,
+                        150, 0.05, 0x1234)
+    assert np.array_equal(ov, g["overlaps"])
+    assert po.state_hash(st) == str(g["hash"])
+
+
+def test_cmpsit_golden(golden):
+    g = golden("cmpsit_n10.npz")
+    hm = synth.hamiltonian(10, 200, seed=6)
+    nat.set_default(nat.FUSED, 9, 2)
+    ov, st = nat.cmpsit(hm, dict(det_idx=np.array([0b0000111100],
+                                                  dtype=np.uint64),
+                                 det_cf=np.array([1.0 + 0j])),
+                        3, 2, 40, 25, 0.05, 0.01, 99)
+    assert np.array_equal(ov, g["overlaps"])
+    assert po.state_hash(st) == str(g["hash"])
+
+
+def test_expand_inner_golden(golden):
+    g = golden("expand.npz")
+    for c in range(int(g["ncases"])):
+        nqb, ns, na, nb, cl, tp = (int(v) for v in g[f"case{c}"])
+        Ca, Cb = g[f"Ca{c}"], g[f"Cb{c}"]
+        print(f"expand case {c}: nqb={nqb} n_sites={ns} na={na} nb={nb} "
+              f"closed={cl} tapered={tp}", flush=True)
+        st = nat.expand(nqb, ns, na, nb, cl, tp, Ca, None if cl else Cb)
+        assert po.state_hash(st) == str(g[f"hash{c}"])
+        r = np.random.default_rng(1000 + c)
+        psi = r.normal(size=1 << nqb) + 1j * r.normal(size=1 << nqb)
+        z = nat.inner(psi, ns, na, nb, cl, tp, Ca, None if cl else Cb)
+        assert close(z, g[f"inner{c}"][0])
+
+
+def test_expand_rejects_out_of_register():
+    C = synth.coeff_matrix(8, 2, seed=1)
+    with pytest.raises(RuntimeError):
+        nat.expand(6, 8, 2, 0, 1, 0, C)  # orbital 7 is beyond 6 qubits
+
+
+# -- phase2.run: the reference's own Python tests, against this backend -----
+
+def test_phase2_run_golden(golden):
+    import phase2_amd
+    g = golden("phase2_run.npz")
+    runs = [(["Z0"], [1.0], 0.3, "0"),
+            (["X0 Y1", "Z1 Z2"], [0.3, -0.5], 0.2, "010"),
+            ([f"X{k}" for k in range(10)], [0.1 * (k + 1) for k in range(10)],
+             0.05, "0" * 10),
+            (list(g["terms8"]), list(g["cf8"]), 0.3, "01101001")]
+    for (p, c, d, s), want in zip(runs, g["out"]):
+        assert phase2_amd.run(p, c, d, s) == want
+
+
+def test_phase2_run_closed_forms():
+    import phase2_amd as phase2
+    TOL = 1e-12
+    r = phase2.run(["Z0"], [1.0], 0.3, "0")
+    assert isinstance(r, complex)
+    assert abs(r - complex(math.cos(0.3), math.sin(0.3))) < TOL
+    r = phase2.run(["Z0"], [1.0], 0.7, "1")
+    assert abs(r - complex(math.cos(0.7), -math.sin(0.7))) < TOL
+    assert abs(phase2.run(["X0"], [1.0], 0.5, "0") - math.cos(0.5)) < TOL
+    assert abs(phase2.run([], [], 1.0, "00") - 1.0) < TOL
+    assert abs(phase2.run(["X0", "X0"], [1.0, 1.0], 0.4, "0").real -
+               math.cos(0.8)) < TOL
+    r = phase2.run(["Z0 Z1"], [1.0], 0.8, "00")
+    assert abs(r - complex(math.cos(0.8), math.sin(0.8))) < TOL
+    h = [0.1 * (k + 1) for k in range(10)]
+    r = phase2.run([f"X{k}" for k in range(10)], h, 0.05, "0" * 10)
+    assert abs(r - math.prod(math.cos(0.05 * v) for v in h)) < TOL
+    psi = "0101010101"
+    r = phase2.run([f"Z{k}" for k in range(10)], h, 0.1, psi)
+    ph = sum((1 if psi[k] == "0" else -1) * 0.1 * h[k] for k in range(10))
+    assert abs(r - complex(math.cos(ph), math.sin(ph))) < TOL
+    assert abs(phase2.run(["Z0"], [1.0], math.pi, "0") + 1.0) < TOL
+    args = (["X0", "Z1"], [0.3, -0.7], 0.25, "01")
+    assert phase2.run(*args) == phase2.run(*args)
+
+
+@pytest.mark.parametrize("bad", [(["Z0"], [1.0, 2.0], 0.1, "0"),
+                                 (["W0"], [1.0], 0.1, "0"),
+                                 (["X5"], [1.0], 0.1, "0"),
+                                 (["Z0"], [1.0], 0.1, "2"),
+                                 (["X"], [1.0], 0.1, "0"),
+                                 (["Z0"], [1.0], 0.1, "0" * 31),
+                                 (["Z0"], [1.0], 0.1, "")])
+def test_phase2_run_errors(bad):
+    import phase2_amd
+    with pytest.raises(ValueError):
+        phase2_amd.run(*bad)
+
+
+# -- against the oracle on fresh seeds ----------------------------------------
+
+@pytest.mark.parametrize("n,K,seed", [(9, 200, 1), (14, 400, 2), (17, 700, 3)])
+def test_rotate_vs_oracle(oracle, n, K, seed):
+    rng = np.random.default_rng(seed)
+    st = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    xm = rng.integers(0, 1 << n, size=K, dtype=np.uint64)
+    zm = rng.integers(0, 1 << n, size=K, dtype=np.uint64)
+    xm[::5] = 0
+    xm[3] = 1 << (n - 1)  # top qubit only
+    phi = rng.normal(size=K)
+    want = oracle.rotate(st, xm, zm, phi)
+    for mode in MODES:
+        if mode[1] > n:
+            continue
+        print(f"rotate n={n} K={K} mode={mode}", flush=True)
+        got = nat.rotate(st, xm, zm, phi, *mode)
+        assert np.array_equal(got, want), mode
+
+
+def test_small_registers_plain_fallback(oracle):
+    for n in (1, 2, 5, 8):
+        rng = np.random.default_rng(n)
+        st = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+        xm = rng.integers(0, 1 << n, size=30, dtype=np.uint64)
+        zm = rng.integers(0, 1 << n, size=30, dtype=np.uint64)
+        phi = rng.normal(size=30)
+        got = nat.rotate(st, xm, zm, phi, nat.FUSED, 12, 6)
+        assert np.array_equal(got, oracle.rotate(st, xm, zm, phi))
+
+
+def test_empty_and_zero_angle():
+    st = np.zeros(1 << 10, dtype=np.complex128)
+    st[5] = 0.6 - 0.8j
+    assert np.array_equal(nat.rotate(st, [], [], []), st)
+    out = nat.rotate(st, [3, 0], [1, 7], [0.0, 0.0])
+    assert np.array_equal(out, st)
+
+
+def test_trott_sampled_vs_oracle(oracle):
+    hm = synth.hamiltonian(15, 500, seed=21, decay=True)
+    idx, cf = synth.multidet_random(15, 5, 6, seed=3)
+    prep = dict(det_idx=idx, det_cf=cf)
+    nat.set_default(nat.FUSED, 12, 6)
+    ov, st = nat.trott(2, hm, prep, 0.07, 3)
+    ov_o, st_o = oracle.trott(2, hm, prep, 0.07, 3)
+    assert np.array_equal(ov, ov_o)
+    assert po.state_hash(st) == po.state_hash(st_o)
+
+
+def test_qdrift_cmpsit_vs_oracle(oracle):
+    hm = synth.hamiltonian(13, 300, seed=8)
+    prep = dict(det_idx=np.array([0b1111000], dtype=np.uint64),
+                det_cf=np.array([1.0 + 0j]))
+    nat.set_default(nat.FUSED, 11, 5)
+    ov, st = nat.qdrift(hm, prep, 3, 400, 0.02, 77)
+    ov_o, st_o = oracle.qdrift(hm, prep, 3, 400, 0.02, 77)
+    assert np.array_equal(ov, ov_o)
+    assert po.state_hash(st) == po.state_hash(st_o)
+    ov, st = nat.cmpsit(hm, prep, 2, 2, 60, 100, 0.04, 0.02, 78)
+    ov_o, st_o = oracle.cmpsit(hm, prep, 2, 2, 60, 100, 0.04, 0.02, 78)
+    assert np.array_equal(ov, ov_o)
+    assert po.state_hash(st) == po.state_hash(st_o)
+
+
+# -- size-independent properties at large registers ----------------------------
+
+def test_fused_equals_plain_n24():
+    """A block of the n=26 workload on 24 qubits (masks with bits >= 24
+    dropped): fused passes leave the state bit-identical to the plain sweep,
+    and undoing the block restores it to 1e-12."""
+    hm = synth.hamiltonian(24, 3000, seed=9, decay=True)
+    rng = np.random.default_rng(5)
+    n = 24
+    st = (rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n))
+    st /= np.linalg.norm(st)
+    phi = 0.05 * hm.cf[:1024] / np.max(np.abs(hm.cf))
+    xm, zm = hm.xm[:1024], hm.zm[:1024]
+    print("n=24 plain", flush=True)
+    a = nat.rotate(st, xm, zm, phi, nat.PLAIN, 12, 6)
+    print("n=24 fused", flush=True)
+    b = nat.rotate(st, xm, zm, phi, nat.FUSED, 12, 6)
+    assert np.array_equal(a, b)
+    c = nat.rotate(st, xm, zm, phi, nat.FUSED, 13, 7)
+    assert np.array_equal(a, c)
+    assert abs(np.linalg.norm(a) - 1.0) < 1e-12
+    back = nat.rotate(a, xm[::-1], zm[::-1], -phi[::-1], nat.FUSED, 12, 6)
+    assert np.max(np.abs(back - st)) < 1e-12
