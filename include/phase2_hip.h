@@ -75,7 +75,7 @@ int ph2hip_rotate_plain(void *amp, uint32_t nqb, const struct ph2hip_rot *rot,
  * PH2HIP_PASS_MAX terms.  No counterpart in the reference. */
 #define PH2HIP_TILE_MIN 9
 #define PH2HIP_TILE_MAX 13
-#define PH2HIP_PASS_MAX 64
+#define PH2HIP_PASS_MAX 48
 int ph2hip_rotate_fused(void *amp, uint32_t nqb, const struct ph2hip_rot *rot,
 	size_t n, const uint32_t *pass_len, const uint32_t *tile_bits,
 	size_t npass, uint32_t low_bits);

@@ -20,6 +20,7 @@
  * into an add; each component update is round(round(c*x) +- round(s*y)), the
  * same as the reference's _Complex double expression evaluated by gcc.
  */
+#define _GNU_SOURCE /* sincos */
 #include <math.h>
 #include <float.h>
 #include <stdint.h>
@@ -90,14 +91,24 @@ static void orc_rotate_one(double *a, uint64_t namp, uint64_t xm,
 	}
 }
 
-/* Sequential rotations; cos/sin from the host libm as in
- * phase2/qreg_qreg.c:189. */
+/* cos and sin of one angle as the reference obtains them: gcc turns the
+ * adjacent cos()/sin() of phase2/qreg_qreg.c:189 into one glibc sincos()
+ * call (it differs from separate calls by one ulp for some angles). */
+static inline void orc_cs(double phi, double *c, double *s)
+{
+	sincos(phi, s, c);
+}
+
+/* Sequential rotations (phase2/qreg_qreg.c:181-198 at one rank). */
 void orc_rotate(double *a, uint32_t nqb, size_t n, const uint64_t *xm,
 	const uint64_t *zm, const double *phi)
 {
 	const uint64_t namp = UINT64_C(1) << nqb;
-	for (size_t k = 0; k < n; k++)
-		orc_rotate_one(a, namp, xm[k], zm[k], cos(phi[k]), sin(phi[k]));
+	double c, s;
+	for (size_t k = 0; k < n; k++) {
+		orc_cs(phi[k], &c, &s);
+		orc_rotate_one(a, namp, xm[k], zm[k], c, s);
+	}
 }
 
 /* ------------------------------------------------------------------ */
@@ -623,7 +634,9 @@ static void run_sweep(struct orc_run *r, const struct orc_term *t, size_t n,
 	for (size_t k = 0; k < n; k++) {
 		const struct orc_term *q = &t[reverse ? n - 1 - k : k];
 		const double phi = omega * q->cf;
-		orc_rotate_one(r->a, r->namp, q->x, q->z, cos(phi), sin(phi));
+		double c, s;
+		orc_cs(phi, &c, &s);
+		orc_rotate_one(r->a, r->namp, q->x, q->z, c, s);
 	}
 }
 
@@ -782,7 +795,9 @@ int orc_phase2_run(uint32_t nqb, uint64_t idx, size_t n, const uint64_t *xm,
 	a[2 * idx] = 1.0;
 	for (size_t k = 0; k < n; k++) {
 		const double phi = delta * coeffs[k];
-		orc_rotate_one(a, namp, xm[k], zm[k], cos(phi), sin(phi));
+		double c, s;
+		orc_cs(phi, &c, &s);
+		orc_rotate_one(a, namp, xm[k], zm[k], c, s);
 	}
 	out[0] = a[2 * idx];
 	out[1] = a[2 * idx + 1];

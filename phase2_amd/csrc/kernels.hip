@@ -113,6 +113,35 @@ __device__ __forceinline__ void rot_pair(double2 &x, double2 &y, int r4,
 	y = ny;
 }
 
+// rot_pair on separate real and imaginary registers.
+__device__ __forceinline__ void rot_pair_s(double &xr, double &xi, double &yr,
+	double &yi, int r4, double c, double s)
+{
+	const bool odd = r4 & 1;
+	const double si = (r4 & 2) ? -s : s;
+	const double sj = ((r4 + 1) & 2) ? -s : s;
+	const double pr = odd ? yr : -yi, pi = odd ? yi : yr;
+	const double qr = odd ? xr : -xi, qi = odd ? xi : xr;
+	const double nxr = add(mul(c, xr), mul(si, pr));
+	const double nxi = add(mul(c, xi), mul(si, pi));
+	const double nyr = add(mul(c, yr), mul(sj, qr));
+	const double nyi = add(mul(c, yi), mul(sj, qi));
+	xr = nxr;
+	xi = nxi;
+	yr = nyr;
+	yi = nyi;
+}
+
+__device__ __forceinline__ void rot_diag_s(double &re, double &im,
+	int odd_parity, double c, double s)
+{
+	const double sj = odd_parity ? -s : s;
+	const double nr = add(mul(c, re), mul(sj, -im));
+	const double ni = add(mul(c, im), mul(sj, re));
+	re = nr;
+	im = ni;
+}
+
 // Diagonal term (x == 0): a <- c a + i (s z) a with z = (-1)^parity.
 __device__ __forceinline__ void rot_diag(double2 &a, int odd_parity,
 	double c, double s)
@@ -185,106 +214,180 @@ __global__ __launch_bounds__(kBlock) void k_rot_diag(
 // fused pass
 // ---------------------------------------------------------------------------
 //
-// A tile is the coset  base ^ span{e_0..e_{L-1}, w_0..w_{g-1}}  of 2^TB
-// amplitudes (TB = L + g).  The w_b have zero low L bits, distinct leading
-// bits p_b (ascending in b) and zeros at every other p_b'.  Tile coordinate
-// v: bits [0,L) are index bits 0..L-1, bit L+b selects w_b.  The tile
-// index T spreads over the index bits that are neither low nor pivots.
-// In these coordinates a term with flip mask x has xt, and
-//   parity(i & z) = parity(base & z) ^ parity(v & zt),
-// and i < i^x iff v < v^xt, so the plain update runs unchanged on v.
+// Tile: the coset  base ^ span{e_0..e_{L-1}, w_0..w_{g-1}}  of 2^TB
+// amplitudes (TB = L + g), held in LDS.  The w_b have zero low L bits,
+// distinct leading bits p_b (ascending in b) and zeros at every other p_b'.
+// Tile coordinate v: bits [0,L) are index bits 0..L-1, bit L+b selects w_b;
+// the tile index spreads over the index bits that are neither low nor
+// pivots.  In these coordinates a term with flip mask x has xt, and
+//   parity(i & z) = parity(base & z) ^ parity(v & zt),   i < i^x <=> v < v^xt,
+// so the plain update runs unchanged on v.
+//
+// Inside the tile the terms of the pass are cut into micro-groups whose xt
+// span at most D dimensions.  A group has its own basis u_0..u_{D-1} (tile
+// coordinates, same echelon form); each thread owns one coset
+//   bu ^ span{u}  (2^D amplitudes, in registers),
+// applies every term of the group to it without communication, and writes
+// it back: one LDS round trip per group instead of one per term.  Within a
+// coset the term's flip pattern xtp (D bits) is a compile-time case, so
+// register indices stay static.
+//
+// LDS slots are XOR-swizzled (bits >= 4 folded into bits 0..3) so that the
+// strided cosets of a wave spread over the banks.
 
 struct FusedTerm {
-	uint64_t z;    // physical phase mask (for the per-tile parity)
-	uint32_t xt;   // flip mask in tile coordinates
-	uint32_t zt;   // phase mask in tile coordinates
+	uint64_t z;    // physical phase mask (per-tile parity)
 	double c, s;
-	uint32_t y;    // popcount(x & z) & 3
-	uint32_t h;    // top bit of xt (unused when xt == 0)
+	uint32_t zt;   // phase mask in tile coordinates
+	uint8_t xtp;   // flip mask in group coordinates
+	uint8_t ztp;   // parity(u_i & zt), group coordinates
+	uint8_t y;     // popcount(x & z) & 3
+	uint8_t pad;
+};
+
+struct FusedGroup {
+	uint16_t u[4];   // group basis, tile coordinates
+	uint8_t piv[4];  // its leading bits, ascending
+	uint8_t first, count, pad[2];
 };
 
 struct FusedPass {
 	uint64_t w[PH2HIP_TILE_MAX];   // gathered basis vectors
-	uint32_t piv[PH2HIP_TILE_MAX]; // their leading bits, ascending
-	uint32_t L, g, nterms, pad;
+	uint8_t piv[PH2HIP_TILE_MAX];  // their leading bits, ascending
+	uint8_t L, g, ngroups, nterms;
+	FusedGroup grp[PH2HIP_PASS_MAX];
 	FusedTerm t[PH2HIP_PASS_MAX];
 };
 
 static_assert(sizeof(FusedPass) <= 4000, "kernel argument too large");
 
-template <int TB>
+__device__ __forceinline__ uint32_t lds_slot(uint32_t v)
+{
+	return v ^ (((v >> 4) ^ (v >> 8)) & 15u);
+}
+
+// One term with compile-time flip pattern XT on a coset of 2^D registers.
+template <int D, int XT>
+__device__ __forceinline__ void coset_term(double (&re)[1 << D],
+	double (&im)[1 << D], uint32_t ztp, uint32_t bpt, uint32_t y, double c,
+	double s)
+{
+	constexpr int H = 31 - __builtin_clz(XT);
+#pragma unroll
+	for (int q = 0; q < (1 << (D - 1)); q++) {
+		const int w = ((q >> H) << (H + 1)) | (q & ((1 << H) - 1));
+		const uint32_t par = bpt ^ (__popc(w & ztp) & 1u);
+		rot_pair_s(re[w], im[w], re[w ^ XT], im[w ^ XT],
+			(int)((y + 2 * par) & 3), c, s);
+	}
+}
+
+template <int D, int XT = 1>
+__device__ __forceinline__ void coset_dispatch(double (&re)[1 << D],
+	double (&im)[1 << D], uint32_t xtp, uint32_t ztp, uint32_t bpt,
+	uint32_t y, double c, double s)
+{
+	if constexpr (XT < (1 << D)) {
+		if (xtp == XT) {
+			coset_term<D, XT>(re, im, ztp, bpt, y, c, s);
+			return;
+		}
+		coset_dispatch<D, XT + 1>(re, im, xtp, ztp, bpt, y, c, s);
+	}
+}
+
+template <int TB, int D>
 __global__ __launch_bounds__(kBlock) void k_rot_fused(
 	double2 *__restrict__ a, const FusedPass P)
 {
 	constexpr uint32_t N = 1u << TB;
 	constexpr uint32_t PER = N / kBlock;          // elements per thread
+	constexpr uint32_t NSUB = N >> D;             // cosets per group
+	static_assert(NSUB % kBlock == 0, "cosets must cover the block");
 	__shared__ double2 tile[N];
 
 	const uint32_t L = P.L, g = P.g;
-	// base: tile index spread over the free index bits
 	uint64_t base = (uint64_t)blockIdx.x << L;
 	for (uint32_t b = 0; b < g; b++)
 		base = insert_zero(base, P.piv[b]);
 
 	const uint32_t lmask = (1u << L) - 1;
 	const uint32_t tid = threadIdx.x;
+	// gathered offset of this thread's row part (rows below 2^(8-L))
+	uint64_t wt = 0;
+	for (uint32_t b = 0; b < g; b++)
+		if ((tid >> L) >> b & 1)
+			wt ^= P.w[b];
 
-	// load: element e = tid + m*256; row = e >> L, col = e & lmask
-#pragma unroll 8
+#pragma unroll 4
 	for (uint32_t m = 0; m < PER; m++) {
 		const uint32_t e = tid + m * kBlock;
-		const uint32_t row = e >> L;
-		uint64_t off = base | (e & lmask);
+		const uint32_t rm = (m * kBlock) >> L;    // uniform row part
+		uint64_t wm = 0;
 		for (uint32_t b = 0; b < g; b++)
-			if (row >> b & 1)
-				off ^= P.w[b];
-		tile[e] = a[off];
+			if (rm >> b & 1)
+				wm ^= P.w[b];
+		tile[lds_slot(e)] = a[base ^ wt ^ wm ^ (e & lmask)];
 	}
-	__syncthreads();
 
-	bool prev_diag = false;
-	for (uint32_t k = 0; k < P.nterms; k++) {
-		const FusedTerm T = P.t[k];
-		const uint32_t bp = __popcll(base & T.z) & 1;
-		if (T.xt == 0) {
-			if (k > 0 && !prev_diag)
-				__syncthreads();
-#pragma unroll 4
-			for (uint32_t m = 0; m < PER; m++) {
-				const uint32_t e = tid + m * kBlock;
-				const int par = (__popc(e & T.zt) & 1) ^ bp;
-				rot_diag(tile[e], par, T.c, T.s);
+	for (uint32_t gi = 0; gi < P.ngroups; gi++) {
+		const FusedGroup G = P.grp[gi];
+		__syncthreads();
+		for (uint32_t j = tid; j < NSUB; j += kBlock) {
+			uint32_t bu = j;
+#pragma unroll
+			for (int i = 0; i < D; i++)
+				bu = insert_zero32(bu, G.piv[i]);
+			double re[1 << D], im[1 << D];
+#pragma unroll
+			for (int w = 0; w < (1 << D); w++) {
+				uint32_t cw = 0;
+#pragma unroll
+				for (int i = 0; i < D; i++)
+					if (w >> i & 1)
+						cw ^= G.u[i];
+				const double2 v = tile[lds_slot(bu ^ cw)];
+				re[w] = v.x;
+				im[w] = v.y;
 			}
-			prev_diag = true;
-		} else {
-			if (k > 0)
-				__syncthreads();
-#pragma unroll 4
-			for (uint32_t m = 0; m < PER / 2; m++) {
-				const uint32_t p = tid + m * kBlock;
-				const uint32_t vi = insert_zero32(p, T.h);
-				const uint32_t vj = vi ^ T.xt;
-				double2 xi = tile[vi], xj = tile[vj];
-				const int par = (__popc(vi & T.zt) & 1) ^ bp;
-				rot_pair(xi, xj, (int)((T.y + 2 * par) & 3), T.c,
-					T.s);
-				tile[vi] = xi;
-				tile[vj] = xj;
+			for (uint32_t k = G.first; k < (uint32_t)G.first + G.count;
+				k++) {
+				const FusedTerm T = P.t[k];
+				const uint32_t bpt = (__popc(bu & T.zt) ^
+					__popcll(base & T.z)) & 1u;
+				if (T.xtp == 0) {
+#pragma unroll
+					for (int w = 0; w < (1 << D); w++)
+						rot_diag_s(re[w], im[w], (int)(bpt ^
+							(__popc(w & T.ztp) & 1u)),
+							T.c, T.s);
+				} else {
+					coset_dispatch<D>(re, im, T.xtp, T.ztp, bpt,
+						T.y, T.c, T.s);
+				}
 			}
-			prev_diag = false;
+#pragma unroll
+			for (int w = 0; w < (1 << D); w++) {
+				uint32_t cw = 0;
+#pragma unroll
+				for (int i = 0; i < D; i++)
+					if (w >> i & 1)
+						cw ^= G.u[i];
+				tile[lds_slot(bu ^ cw)] = make_double2(re[w], im[w]);
+			}
 		}
 	}
 	__syncthreads();
 
-#pragma unroll 8
+#pragma unroll 4
 	for (uint32_t m = 0; m < PER; m++) {
 		const uint32_t e = tid + m * kBlock;
-		const uint32_t row = e >> L;
-		uint64_t off = base | (e & lmask);
+		const uint32_t rm = (m * kBlock) >> L;
+		uint64_t wm = 0;
 		for (uint32_t b = 0; b < g; b++)
-			if (row >> b & 1)
-				off ^= P.w[b];
-		a[off] = tile[e];
+			if (rm >> b & 1)
+				wm ^= P.w[b];
+		a[base ^ wt ^ wm ^ (e & lmask)] = tile[lds_slot(e)];
 	}
 }
 
@@ -530,6 +633,13 @@ uint32_t reduce_basis(const struct ph2hip_rot *rot, size_t n, uint64_t lowmask,
 	return g;
 }
 
+// Micro-group depth per tile size: one coset of 2^D per thread per group
+// (2^(TB-D) = 256 cosets for TB <= 12; 512 for TB = 13).
+constexpr int group_depth(int tb)
+{
+	return tb >= 12 ? 4 : tb - 8;
+}
+
 template <int TB>
 int launch_fused(double2 *a, uint32_t nqb, const FusedPass &P)
 {
@@ -537,10 +647,33 @@ int launch_fused(double2 *a, uint32_t nqb, const FusedPass &P)
 	hipEvent_t stop;
 	if (prof_begin(2, P.nterms, &stop))
 		return -1;
-	hipLaunchKernelGGL(k_rot_fused<TB>, dim3((uint32_t)ntiles),
-		dim3(kBlock), 0, g_stream, a, P);
+	hipLaunchKernelGGL((k_rot_fused<TB, group_depth(TB)>),
+		dim3((uint32_t)ntiles), dim3(kBlock), 0, g_stream, a, P);
 	HIPCHK(hipGetLastError());
 	return prof_end(stop);
+}
+
+// Echelon basis of span{v} for tile-coordinate vectors (leading-bit pivots,
+// each basis vector zero at the other pivots); returns the rank or cap+1.
+uint32_t reduce32(const uint32_t *v, size_t n, uint32_t *u, uint32_t cap)
+{
+	uint32_t d = 0;
+	for (size_t k = 0; k < n; k++) {
+		uint32_t x = v[k];
+		for (uint32_t i = 0; i < d && x; i++)
+			if (x >> (31 - __builtin_clz(u[i])) & 1)
+				x ^= u[i];
+		if (!x)
+			continue;
+		if (d == cap)
+			return cap + 1;
+		const int p = 31 - __builtin_clz(x);
+		for (uint32_t i = 0; i < d; i++)
+			if (u[i] >> p & 1)
+				u[i] ^= x;
+		u[d++] = x;
+	}
+	return d;
 }
 
 } // namespace
@@ -762,40 +895,99 @@ int ph2hip_rotate_fused(void *amp, uint32_t nqb, const struct ph2hip_rot *rot,
 				break;
 			L = tb - g;
 		}
-		P.L = L;
-		P.g = g;
-		P.nterms = (uint32_t)len;
+		P.L = (uint8_t)L;
+		P.g = (uint8_t)g;
+		P.nterms = (uint8_t)len;
 		for (uint32_t b = 0; b < g; b++) {
 			P.w[b] = w[b];
-			P.piv[b] = (uint32_t)top_bit(w[b]);
+			P.piv[b] = (uint8_t)top_bit(w[b]);
 		}
 		const uint64_t lm = (UINT64_C(1) << L) - 1;
+		uint32_t xt[PH2HIP_PASS_MAX];
 		for (size_t k = 0; k < len; k++) {
 			const uint64_t x = r[k].x, z = r[k].z;
-			uint32_t xt = (uint32_t)(x & lm), zt = (uint32_t)(z & lm);
+			uint32_t xk = (uint32_t)(x & lm), zk = (uint32_t)(z & lm);
 			uint64_t rest = x & ~lm;
 			for (uint32_t b = 0; b < g; b++) {
 				if (x >> P.piv[b] & 1) {
-					xt |= 1u << (L + b);
+					xk |= 1u << (L + b);
 					rest ^= w[b];
 				}
 				if (__builtin_popcountll(w[b] & z) & 1)
-					zt |= 1u << (L + b);
+					zk |= 1u << (L + b);
 			}
 			if (rest) {
 				free(passes);
 				return fail("rotate_fused: term %zu outside its tile",
 					k0 + k);
 			}
+			xt[k] = xk;
 			FusedTerm &T = P.t[k];
 			T.z = z;
-			T.xt = xt;
-			T.zt = zt;
+			T.zt = zk;
 			T.c = r[k].c;
 			T.s = r[k].s;
-			T.y = __builtin_popcountll(x & z) & 3;
-			T.h = xt ? (uint32_t)(31 - __builtin_clz(xt)) : 0;
+			T.y = (uint8_t)(__builtin_popcountll(x & z) & 3);
 		}
+		// micro-groups: consecutive terms whose xt span <= D dims
+		const uint32_t D = (uint32_t)group_depth((int)tb);
+		uint32_t ng = 0;
+		size_t k = 0;
+		while (k < len) {
+			size_t e = k + 1;
+			uint32_t u[8];
+			while (e < len && reduce32(xt + k, e + 1 - k, u, D) <= D)
+				e++;
+			uint32_t d = reduce32(xt + k, e - k, u, D);
+			// pad with unit vectors of tile bits outside the span
+			for (uint32_t bit = 0; d < D && bit < tb; bit++) {
+				uint32_t x = 1u << bit;
+				for (uint32_t i = 0; i < d && x; i++)
+					if (x >> (31 - __builtin_clz(u[i])) & 1)
+						x ^= u[i];
+				if (!x)
+					continue;
+				const int p = 31 - __builtin_clz(x);
+				for (uint32_t i = 0; i < d; i++)
+					if (u[i] >> p & 1)
+						u[i] ^= x;
+				u[d++] = x;
+			}
+			for (uint32_t i = 1; i < d; i++)   // ascending pivots
+				for (uint32_t j = i; j > 0 && __builtin_clz(u[j - 1]) <
+					__builtin_clz(u[j]); j--) {
+					const uint32_t t = u[j];
+					u[j] = u[j - 1];
+					u[j - 1] = t;
+				}
+			FusedGroup &G = P.grp[ng++];
+			memset(&G, 0, sizeof G);
+			for (uint32_t i = 0; i < D; i++) {
+				G.u[i] = (uint16_t)u[i];
+				G.piv[i] = (uint8_t)(31 - __builtin_clz(u[i]));
+			}
+			G.first = (uint8_t)k;
+			G.count = (uint8_t)(e - k);
+			for (size_t q = k; q < e; q++) {
+				uint32_t xp = 0, zp = 0, rest = xt[q];
+				for (uint32_t i = 0; i < D; i++) {
+					if (xt[q] >> G.piv[i] & 1) {
+						xp |= 1u << i;
+						rest ^= u[i];
+					}
+					if (__builtin_popcount(u[i] & P.t[q].zt) & 1)
+						zp |= 1u << i;
+				}
+				if (rest) {
+					free(passes);
+					return fail("rotate_fused: group basis");
+				}
+				P.t[q].xtp = (uint8_t)xp;
+				P.t[q].ztp = (uint8_t)zp;
+			}
+			k = e;
+		}
+		P.ngroups = (uint8_t)ng;
 		k0 += len;
 	}
 	double2 *a = (double2 *)amp;

@@ -7,11 +7,15 @@
  * the reference's single-rank flush (self-exchange, mix with bm = 1, rotate,
  * add 0) reduces exactly to in-order rotations, which is what
  * qreg_paulirot applies here.  cos/sin come from the host libm per term as
- * in phase2/qreg_qreg.c:189.
+ * in phase2/qreg_qreg.c:189, through sincos(): gcc folds the reference's
+ * adjacent cos() and sin() of one angle into a single glibc sincos() call,
+ * whose results differ from separate cos()/sin() calls by one ulp for some
+ * angles, so calling sincos() explicitly is what reproduces the reference.
  *
  * The void entry points cannot return errors; a failed device call logs
  * and sets a sticky flag read by qreg_error().
  */
+#define _GNU_SOURCE /* sincos */
 #define LOG_SUBSYS "qreg"
 
 #include <complex.h>
@@ -245,8 +249,7 @@ void qreg_paulirot(struct qreg *reg, struct paulis code_hi,
 	for (size_t k = 0; k < ncodes; k++) {
 		h->rot[k].x = codes_lo[k].pak[0];
 		h->rot[k].z = codes_lo[k].pak[1];
-		h->rot[k].c = cos(phis[k]);
-		h->rot[k].s = sin(phis[k]);
+		sincos(phis[k], &h->rot[k].s, &h->rot[k].c);
 	}
 	const uint32_t nqb = reg->qb_lo;
 	int rc;
