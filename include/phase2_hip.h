@@ -65,6 +65,10 @@ struct ph2hip_rot {
 int ph2hip_rotate_plain(void *amp, uint32_t nqb, const struct ph2hip_rot *rot,
 	size_t n);
 
+/* Plain-kernel variant: 2, 4 or 8 pairs per thread, plus 16 for
+ * non-temporal stores (default 2).  All variants give the same state. */
+int ph2hip_set_plain_form(int form);
+
 /* Fused sweep: the terms are applied in order, pass by pass; pass p covers
  * the next pass_len[p] terms and runs as one HBM pass over LDS tiles of
  * 2^tile_bits[p] amplitudes, each tile a coset of a subspace that contains
@@ -79,6 +83,12 @@ int ph2hip_rotate_plain(void *amp, uint32_t nqb, const struct ph2hip_rot *rot,
 int ph2hip_rotate_fused(void *amp, uint32_t nqb, const struct ph2hip_rot *rot,
 	size_t n, const uint32_t *pass_len, const uint32_t *tile_bits,
 	size_t npass, uint32_t low_bits);
+
+/* Kernel form of the fused pass per tile size: bit TB of pipe_mask selects
+ * the persistent, software-pipelined form (persistent_wgs workgroups; 0
+ * keeps the current count, initially two per compute unit) instead of one
+ * workgroup per tile.  Initial mask from PHASE2_HIP_PIPE, else 0. */
+int ph2hip_set_fused_form(uint32_t pipe_mask, uint64_t persistent_wgs);
 
 /* Dimension of span{x_k & ~(2^low_bits - 1)} over GF(2) for k in [0, n):
  * the number of tile bits beyond low_bits a pass over these terms needs. */
@@ -127,6 +137,22 @@ struct ph2hip_prof {
 };
 int ph2hip_prof_enable(int on);
 int ph2hip_prof_read(struct ph2hip_prof *out, int reset);
+
+/* Per-launch records kept while profiling is enabled (the newest
+ * PH2HIP_REC_MAX).  For fused passes: tile bits, contiguous low bits L,
+ * gathered dimensions g, terms, micro-groups, the index bits the tile
+ * gathers (pivot mask), the union of the pass's flip and phase masks, and
+ * the kernel time.  ph2hip_prof_records copies up to cap records (oldest
+ * first, after draining pending events) and returns how many, or -1. */
+#define PH2HIP_REC_MAX 65536
+struct ph2hip_launch_rec {
+	uint32_t kind, nqb, tile_bits, low_bits;
+	uint32_t gdim, nterms, ngroups, pad;
+	uint64_t pivots, xunion, zunion;
+	double ms;
+};
+long ph2hip_prof_records(struct ph2hip_launch_rec *out, size_t cap,
+	int reset);
 
 /* Events on the library stream for caller-side timing. */
 int ph2hip_mark(int slot);

@@ -332,3 +332,33 @@ class Register:
             self.free()
         except Exception:
             pass
+
+
+REC_DTYPE = np.dtype([
+    ("kind", "<u4"), ("nqb", "<u4"), ("tile_bits", "<u4"), ("low_bits", "<u4"),
+    ("gdim", "<u4"), ("nterms", "<u4"), ("ngroups", "<u4"), ("pad", "<u4"),
+    ("pivots", "<u8"), ("xunion", "<u8"), ("zunion", "<u8"), ("ms", "<f8")])
+
+
+def prof_records(reset=True, cap=65536) -> np.ndarray:
+    """Per-launch records (struct ph2hip_launch_rec) kept while profiling."""
+    out = np.zeros(cap, dtype=REC_DTYPE)
+    L = lib()
+    L.ph2hip_prof_records.restype = C.c_long
+    n = L.ph2hip_prof_records(out.ctypes.data_as(C.c_void_p), C.c_size_t(cap),
+                              C.c_int(int(reset)))
+    if n < 0:
+        _err("ph2hip_prof_records")
+    return out[:n].copy()
+
+
+def set_fused_form(pipe_mask: int, persistent_wgs: int = 0):
+    """Bit TB of pipe_mask: persistent pipelined fused kernel for tile TB."""
+    if lib().ph2hip_set_fused_form(C.c_uint32(pipe_mask),
+                                   C.c_uint64(persistent_wgs)) != 0:
+        _err("ph2hip_set_fused_form")
+
+
+def set_plain_form(form: int):
+    """2, 4 or 8 pairs per thread; +16 for non-temporal stores."""
+    lib().ph2hip_set_plain_form(C.c_int(form))

@@ -41,6 +41,7 @@ namespace {
 char g_err[256] = "";
 hipStream_t g_stream = nullptr;
 int g_device = -1;
+uint64_t g_persist_wgs = 512;   // fused-pass workgroups (2 per CU by default)
 double *g_red = nullptr;       // reduction partials (device)
 size_t g_red_cap = 0;
 
@@ -113,35 +114,6 @@ __device__ __forceinline__ void rot_pair(double2 &x, double2 &y, int r4,
 	y = ny;
 }
 
-// rot_pair on separate real and imaginary registers.
-__device__ __forceinline__ void rot_pair_s(double &xr, double &xi, double &yr,
-	double &yi, int r4, double c, double s)
-{
-	const bool odd = r4 & 1;
-	const double si = (r4 & 2) ? -s : s;
-	const double sj = ((r4 + 1) & 2) ? -s : s;
-	const double pr = odd ? yr : -yi, pi = odd ? yi : yr;
-	const double qr = odd ? xr : -xi, qi = odd ? xi : xr;
-	const double nxr = add(mul(c, xr), mul(si, pr));
-	const double nxi = add(mul(c, xi), mul(si, pi));
-	const double nyr = add(mul(c, yr), mul(sj, qr));
-	const double nyi = add(mul(c, yi), mul(sj, qi));
-	xr = nxr;
-	xi = nxi;
-	yr = nyr;
-	yi = nyi;
-}
-
-__device__ __forceinline__ void rot_diag_s(double &re, double &im,
-	int odd_parity, double c, double s)
-{
-	const double sj = odd_parity ? -s : s;
-	const double nr = add(mul(c, re), mul(sj, -im));
-	const double ni = add(mul(c, im), mul(sj, re));
-	re = nr;
-	im = ni;
-}
-
 // Diagonal term (x == 0): a <- c a + i (s z) a with z = (-1)^parity.
 __device__ __forceinline__ void rot_diag(double2 &a, int odd_parity,
 	double c, double s)
@@ -159,7 +131,18 @@ constexpr int kBlock = 256;
 
 // Pair t -> i = t with a zero inserted at bit h (the top bit of x), j = i^x
 // > i.  Consecutive lanes take consecutive t, so both streams coalesce.
-template <int UNR>
+template <bool NT>
+__device__ __forceinline__ void st16(double2 *p, double2 v)
+{
+	if (NT) {
+		__builtin_nontemporal_store(v.x, &p->x);
+		__builtin_nontemporal_store(v.y, &p->y);
+	} else {
+		*p = v;
+	}
+}
+
+template <int UNR, bool NT>
 __global__ __launch_bounds__(kBlock) void k_rot_pair(
 	double2 *__restrict__ a, uint64_t npairs, uint64_t x, uint64_t z,
 	uint32_t h, uint32_t y, double c, double s)
@@ -182,13 +165,13 @@ __global__ __launch_bounds__(kBlock) void k_rot_pair(
 		if (t < npairs) {
 			const int par = __popcll(ii[u] & z) & 1;
 			rot_pair(vi[u], vj[u], (int)((y + 2 * par) & 3), c, s);
-			a[ii[u]] = vi[u];
-			a[ii[u] ^ x] = vj[u];
+			st16<NT>(&a[ii[u]], vi[u]);
+			st16<NT>(&a[ii[u] ^ x], vj[u]);
 		}
 	}
 }
 
-template <int UNR>
+template <int UNR, bool NT>
 __global__ __launch_bounds__(kBlock) void k_rot_diag(
 	double2 *__restrict__ a, uint64_t namp, uint64_t z, double c, double s)
 {
@@ -205,7 +188,7 @@ __global__ __launch_bounds__(kBlock) void k_rot_diag(
 		const uint64_t i = i0 + (uint64_t)u * kBlock;
 		if (i < namp) {
 			rot_diag(v[u], __popcll(i & z) & 1, c, s);
-			a[i] = v[u];
+			st16<NT>(&a[i], v[u]);
 		}
 	}
 }
@@ -248,11 +231,15 @@ struct FusedTerm {
 struct FusedGroup {
 	uint16_t u[4];   // group basis, tile coordinates
 	uint8_t piv[4];  // its leading bits, ascending
+	uint8_t jpos[12];// tile bit of each coset-index bit (the free bits)
 	uint8_t first, count, pad[2];
 };
 
+constexpr int kMaxPer = (1 << PH2HIP_TILE_MAX) / kBlock;
+
 struct FusedPass {
 	uint64_t w[PH2HIP_TILE_MAX];   // gathered basis vectors
+	uint64_t roff[kMaxPer];        // gathered offset of load/store step m
 	uint8_t piv[PH2HIP_TILE_MAX];  // their leading bits, ascending
 	uint8_t L, g, ngroups, nterms;
 	FusedGroup grp[PH2HIP_PASS_MAX];
@@ -261,83 +248,98 @@ struct FusedPass {
 
 static_assert(sizeof(FusedPass) <= 4000, "kernel argument too large");
 
+// LDS slot of tile coordinate v: bits >= 4 folded into bits 0..3.  The map
+// is GF(2)-linear, so slot(a ^ b) = slot(a) ^ slot(b).
+typedef double v2d __attribute__((ext_vector_type(2)));
+
 __device__ __forceinline__ uint32_t lds_slot(uint32_t v)
 {
 	return v ^ (((v >> 4) ^ (v >> 8)) & 15u);
 }
 
+__device__ __forceinline__ double flip_sign(double v, uint32_t f)
+{
+	return __hiloint2double(__double2hiint(v) ^ (int)(f << 31),
+		__double2loint(v));
+}
+
 // One term with compile-time flip pattern XT on a coset of 2^D registers.
-template <int D, int XT>
+// r4 = y + 2*par (mod 4) with par = bpt ^ parity(w & ztp).  Odd y: the
+// partner enters as itself and s_j = -s_i; even y: as i*partner and
+// s_j = s_i.  s_i = s with its sign flipped by par ^ bit1(y); sb already
+// carries bpt ^ bit1(y), so per pair only parity(w & ztp) (uniform) flips.
+template <int D, int XT, bool ODD>
 __device__ __forceinline__ void coset_term(double (&re)[1 << D],
-	double (&im)[1 << D], uint32_t ztp, uint32_t bpt, uint32_t y, double c,
-	double s)
+	double (&im)[1 << D], uint32_t ztp, double sb, double c)
 {
 	constexpr int H = 31 - __builtin_clz(XT);
 #pragma unroll
 	for (int q = 0; q < (1 << (D - 1)); q++) {
 		const int w = ((q >> H) << (H + 1)) | (q & ((1 << H) - 1));
-		const uint32_t par = bpt ^ (__popc(w & ztp) & 1u);
-		rot_pair_s(re[w], im[w], re[w ^ XT], im[w ^ XT],
-			(int)((y + 2 * par) & 3), c, s);
+		const double si = flip_sign(sb, __popc(w & ztp) & 1u);
+		double &xr = re[w], &xi = im[w];
+		double &yr = re[w ^ XT], &yi = im[w ^ XT];
+		double nxr, nxi, nyr, nyi;
+		if (ODD) {
+			nxr = add(mul(c, xr), mul(si, yr));
+			nxi = add(mul(c, xi), mul(si, yi));
+			nyr = add(mul(c, yr), mul(-si, xr));
+			nyi = add(mul(c, yi), mul(-si, xi));
+		} else {
+			nxr = add(mul(c, xr), mul(si, -yi));
+			nxi = add(mul(c, xi), mul(si, yr));
+			nyr = add(mul(c, yr), mul(si, -xi));
+			nyi = add(mul(c, yi), mul(si, xr));
+		}
+		xr = nxr;
+		xi = nxi;
+		yr = nyr;
+		yi = nyi;
 	}
 }
 
-template <int D, int XT = 1>
+template <int D, bool ODD, int XT = 1>
 __device__ __forceinline__ void coset_dispatch(double (&re)[1 << D],
-	double (&im)[1 << D], uint32_t xtp, uint32_t ztp, uint32_t bpt,
-	uint32_t y, double c, double s)
+	double (&im)[1 << D], uint32_t xtp, uint32_t ztp, double sb, double c)
 {
 	if constexpr (XT < (1 << D)) {
 		if (xtp == XT) {
-			coset_term<D, XT>(re, im, ztp, bpt, y, c, s);
+			coset_term<D, XT, ODD>(re, im, ztp, sb, c);
 			return;
 		}
-		coset_dispatch<D, XT + 1>(re, im, xtp, ztp, bpt, y, c, s);
+		coset_dispatch<D, ODD, XT + 1>(re, im, xtp, ztp, sb, c);
 	}
 }
 
-template <int TB, int D>
-__global__ __launch_bounds__(kBlock) void k_rot_fused(
-	double2 *__restrict__ a, const FusedPass P)
+__device__ __forceinline__ uint64_t tile_base(const FusedPass &P, uint64_t t)
 {
-	constexpr uint32_t N = 1u << TB;
-	constexpr uint32_t PER = N / kBlock;          // elements per thread
-	constexpr uint32_t NSUB = N >> D;             // cosets per group
-	static_assert(NSUB % kBlock == 0, "cosets must cover the block");
-	__shared__ double2 tile[N];
-
-	const uint32_t L = P.L, g = P.g;
-	uint64_t base = (uint64_t)blockIdx.x << L;
-	for (uint32_t b = 0; b < g; b++)
+	uint64_t base = t << P.L;
+	for (uint32_t b = 0; b < P.g; b++)
 		base = insert_zero(base, P.piv[b]);
+	return base;
+}
 
-	const uint32_t lmask = (1u << L) - 1;
-	const uint32_t tid = threadIdx.x;
-	// gathered offset of this thread's row part (rows below 2^(8-L))
-	uint64_t wt = 0;
-	for (uint32_t b = 0; b < g; b++)
-		if ((tid >> L) >> b & 1)
-			wt ^= P.w[b];
-
-#pragma unroll 4
-	for (uint32_t m = 0; m < PER; m++) {
-		const uint32_t e = tid + m * kBlock;
-		const uint32_t rm = (m * kBlock) >> L;    // uniform row part
-		uint64_t wm = 0;
-		for (uint32_t b = 0; b < g; b++)
-			if (rm >> b & 1)
-				wm ^= P.w[b];
-		tile[lds_slot(e)] = a[base ^ wt ^ wm ^ (e & lmask)];
-	}
-
+// Persistent, software-pipelined pass: each workgroup walks tiles
+// blockIdx.x, +gridDim.x, ...; the next tile's HBM reads are in flight in
+// registers while the current tile is rotated in LDS, and its writes are
+// issued without waiting.
+// All micro-groups of a pass on one tile held in LDS.
+template <int TB, int D>
+__device__ __forceinline__ void tile_groups(v2d *tile, const FusedPass &P,
+	uint64_t base, uint32_t tid)
+{
+	constexpr uint32_t NSUB = (1u << TB) >> D;    // cosets per group
+	static_assert(NSUB % kBlock == 0, "cosets must cover the block");
 	for (uint32_t gi = 0; gi < P.ngroups; gi++) {
 		const FusedGroup G = P.grp[gi];
 		__syncthreads();
 		for (uint32_t j = tid; j < NSUB; j += kBlock) {
-			uint32_t bu = j;
+			// coset base: j's bits spread over the free tile bits
+			uint32_t bu = 0;
 #pragma unroll
-			for (int i = 0; i < D; i++)
-				bu = insert_zero32(bu, G.piv[i]);
+			for (int k = 0; k < TB - D; k++)
+				bu |= (j >> k & 1u) << G.jpos[k];
+			const uint32_t sbu = lds_slot(bu);
 			double re[1 << D], im[1 << D];
 #pragma unroll
 			for (int w = 0; w < (1 << D); w++) {
@@ -346,7 +348,7 @@ __global__ __launch_bounds__(kBlock) void k_rot_fused(
 				for (int i = 0; i < D; i++)
 					if (w >> i & 1)
 						cw ^= G.u[i];
-				const double2 v = tile[lds_slot(bu ^ cw)];
+				const v2d v = tile[sbu ^ lds_slot(cw)];
 				re[w] = v.x;
 				im[w] = v.y;
 			}
@@ -355,15 +357,30 @@ __global__ __launch_bounds__(kBlock) void k_rot_fused(
 				const FusedTerm T = P.t[k];
 				const uint32_t bpt = (__popc(bu & T.zt) ^
 					__popcll(base & T.z)) & 1u;
+				const double c = T.c;
 				if (T.xtp == 0) {
+					// diagonal: a <- c a + s_j (i a), s_j = +-s
+					const double sb = flip_sign(T.s, bpt);
 #pragma unroll
-					for (int w = 0; w < (1 << D); w++)
-						rot_diag_s(re[w], im[w], (int)(bpt ^
-							(__popc(w & T.ztp) & 1u)),
-							T.c, T.s);
+					for (int w = 0; w < (1 << D); w++) {
+						const double sj = flip_sign(sb,
+							__popc(w & T.ztp) & 1u);
+						const double nr = add(mul(c, re[w]),
+							mul(sj, -im[w]));
+						const double ni = add(mul(c, im[w]),
+							mul(sj, re[w]));
+						re[w] = nr;
+						im[w] = ni;
+					}
 				} else {
-					coset_dispatch<D>(re, im, T.xtp, T.ztp, bpt,
-						T.y, T.c, T.s);
+					const double sb = flip_sign(T.s,
+						bpt ^ ((T.y >> 1) & 1u));
+					if (T.y & 1)
+						coset_dispatch<D, true>(re, im, T.xtp,
+							T.ztp, sb, c);
+					else
+						coset_dispatch<D, false>(re, im, T.xtp,
+							T.ztp, sb, c);
 				}
 			}
 #pragma unroll
@@ -373,21 +390,87 @@ __global__ __launch_bounds__(kBlock) void k_rot_fused(
 				for (int i = 0; i < D; i++)
 					if (w >> i & 1)
 						cw ^= G.u[i];
-				tile[lds_slot(bu ^ cw)] = make_double2(re[w], im[w]);
+				v2d v;
+				v.x = re[w];
+				v.y = im[w];
+				tile[sbu ^ lds_slot(cw)] = v;
 			}
 		}
 	}
 	__syncthreads();
+}
 
+// One tile per workgroup (grid = number of tiles); HBM latency is hidden
+// by the co-resident workgroups of the CU.
+template <int TB, int D>
+__global__ __launch_bounds__(kBlock) void k_rot_fused(
+	double2 *__restrict__ amp, const FusedPass P)
+{
+	constexpr uint32_t PER = (1u << TB) / kBlock;
+	__shared__ v2d tile[1u << TB];
+	// clang vector type: plain 16-byte accesses that stay in registers
+	v2d *__restrict__ a = reinterpret_cast<v2d *>(amp);
+	const uint32_t L = P.L, lmask = (1u << L) - 1, tid = threadIdx.x;
+	uint64_t wt = 0;   // gathered offset of this thread's row part
+	for (uint32_t b = 0; b < P.g; b++)
+		if ((tid >> L) >> b & 1)
+			wt ^= P.w[b];
+	const uint64_t base = tile_base(P, blockIdx.x);
+	const uint64_t bo = base ^ wt;
 #pragma unroll 4
-	for (uint32_t m = 0; m < PER; m++) {
-		const uint32_t e = tid + m * kBlock;
-		const uint32_t rm = (m * kBlock) >> L;
-		uint64_t wm = 0;
-		for (uint32_t b = 0; b < g; b++)
-			if (rm >> b & 1)
-				wm ^= P.w[b];
-		a[base ^ wt ^ wm ^ (e & lmask)] = tile[lds_slot(e)];
+	for (uint32_t m = 0; m < PER; m++)
+		tile[lds_slot(tid + m * kBlock)] =
+			a[bo ^ P.roff[m] ^ ((tid + m * kBlock) & lmask)];
+	tile_groups<TB, D>(tile, P, base, tid);
+#pragma unroll
+	for (uint32_t m = 0; m < PER; m++)
+		a[bo ^ P.roff[m] ^ ((tid + m * kBlock) & lmask)] =
+			tile[lds_slot(tid + m * kBlock)];
+}
+
+// Persistent, software-pipelined form: each workgroup walks tiles
+// blockIdx.x, +gridDim.x, ...; the next tile's HBM reads are in flight in
+// registers while the current tile is rotated in LDS.
+template <int TB, int D>
+__global__ __launch_bounds__(kBlock) void k_rot_fused_pipe(
+	double2 *__restrict__ amp, const FusedPass P, uint64_t ntiles)
+{
+	constexpr uint32_t PER = (1u << TB) / kBlock;
+	__shared__ v2d tile[1u << TB];
+	v2d *__restrict__ a = reinterpret_cast<v2d *>(amp);
+	const uint32_t L = P.L, lmask = (1u << L) - 1, tid = threadIdx.x;
+	uint64_t wt = 0;
+	for (uint32_t b = 0; b < P.g; b++)
+		if ((tid >> L) >> b & 1)
+			wt ^= P.w[b];
+	v2d pre[PER];
+	uint64_t t = blockIdx.x;
+	if (t < ntiles) {
+		const uint64_t bn = tile_base(P, t) ^ wt;
+#pragma unroll
+		for (uint32_t m = 0; m < PER; m++)
+			pre[m] = a[bn ^ P.roff[m] ^ ((tid + m * kBlock) & lmask)];
+	}
+	for (; t < ntiles; t += gridDim.x) {
+		const uint64_t base = tile_base(P, t);
+		__syncthreads();          // previous tile fully written out
+#pragma unroll
+		for (uint32_t m = 0; m < PER; m++)
+			tile[lds_slot(tid + m * kBlock)] = pre[m];
+		const uint64_t tn = t + gridDim.x;
+		if (tn < ntiles) {
+			const uint64_t bn = tile_base(P, tn) ^ wt;
+#pragma unroll
+			for (uint32_t m = 0; m < PER; m++)
+				pre[m] = a[bn ^ P.roff[m] ^
+					((tid + m * kBlock) & lmask)];
+		}
+		tile_groups<TB, D>(tile, P, base, tid);
+		const uint64_t bo = base ^ wt;
+#pragma unroll
+		for (uint32_t m = 0; m < PER; m++)
+			a[bo ^ P.roff[m] ^ ((tid + m * kBlock) & lmask)] =
+				tile[lds_slot(tid + m * kBlock)];
 	}
 }
 
@@ -534,6 +617,9 @@ int g_pkind[kProfPool];
 uint32_t g_pterms[kProfPool];
 int g_pn = 0, g_pev_made = 0;
 ph2hip_prof g_pacc = {};
+ph2hip_launch_rec g_pmeta[kProfPool];   // shape of each pending launch
+ph2hip_launch_rec *g_rec = nullptr;     // ring of finished records
+size_t g_rec_n = 0, g_rec_head = 0;
 hipEvent_t g_marks[8];
 bool g_marks_made = false;
 
@@ -546,13 +632,27 @@ int prof_drain()
 		g_pacc.launches[g_pkind[i]]++;
 		g_pacc.ms[g_pkind[i]] += ms;
 		g_pacc.terms[g_pkind[i]] += g_pterms[i];
+		if (!g_rec) {
+			g_rec = (ph2hip_launch_rec *)calloc(PH2HIP_REC_MAX,
+				sizeof *g_rec);
+			if (!g_rec)
+				return fail("prof: out of host memory");
+		}
+		ph2hip_launch_rec r = g_pmeta[i];
+		r.ms = ms;
+		g_rec[(g_rec_head + g_rec_n) % PH2HIP_REC_MAX] = r;
+		if (g_rec_n < PH2HIP_REC_MAX)
+			g_rec_n++;
+		else
+			g_rec_head = (g_rec_head + 1) % PH2HIP_REC_MAX;
 	}
 	g_pn = 0;
 	return 0;
 }
 
 // Event pair for the next launch (nullptr events when profiling is off).
-int prof_begin(int kind, uint32_t nterms, hipEvent_t *stop)
+int prof_begin(int kind, uint32_t nterms, hipEvent_t *stop,
+	const ph2hip_launch_rec *meta = nullptr)
 {
 	*stop = nullptr;
 	if (!g_prof)
@@ -566,6 +666,12 @@ int prof_begin(int kind, uint32_t nterms, hipEvent_t *stop)
 	}
 	g_pkind[g_pn] = kind;
 	g_pterms[g_pn] = nterms;
+	if (meta)
+		g_pmeta[g_pn] = *meta;
+	else
+		g_pmeta[g_pn] = ph2hip_launch_rec{};
+	g_pmeta[g_pn].kind = (uint32_t)kind;
+	g_pmeta[g_pn].nterms = nterms;
 	HIPCHK(hipEventRecord(g_pev[g_pn][0], g_stream));
 	*stop = g_pev[g_pn][1];
 	g_pn++;
@@ -640,15 +746,42 @@ constexpr int group_depth(int tb)
 	return tb >= 12 ? 4 : tb - 8;
 }
 
+bool g_pipe_init = false;
+uint32_t g_pipe_mask = 0;   // bit TB set: persistent pipelined form
+
 template <int TB>
-int launch_fused(double2 *a, uint32_t nqb, const FusedPass &P)
+int launch_fused(double2 *a, uint32_t nqb, const FusedPass &P,
+	uint64_t xunion, uint64_t zunion)
 {
+	if (!g_pipe_init) {
+		const char *e = getenv("PHASE2_HIP_PIPE");
+		g_pipe_mask = e ? (uint32_t)strtoul(e, nullptr, 0) : 0;
+		g_pipe_init = true;
+	}
 	const uint64_t ntiles = UINT64_C(1) << (nqb - TB);
 	hipEvent_t stop;
-	if (prof_begin(2, P.nterms, &stop))
+	ph2hip_launch_rec meta{};
+	meta.nqb = nqb;
+	meta.tile_bits = TB;
+	meta.low_bits = P.L;
+	meta.gdim = P.g;
+	meta.ngroups = P.ngroups;
+	for (uint32_t b = 0; b < P.g; b++)
+		meta.pivots |= UINT64_C(1) << P.piv[b];
+	meta.xunion = xunion;
+	meta.zunion = zunion;
+	if (prof_begin(2, P.nterms, &stop, &meta))
 		return -1;
-	hipLaunchKernelGGL((k_rot_fused<TB, group_depth(TB)>),
-		dim3((uint32_t)ntiles), dim3(kBlock), 0, g_stream, a, P);
+	if (g_pipe_mask >> TB & 1) {
+		const uint64_t grid = ntiles < g_persist_wgs ? ntiles
+			: g_persist_wgs;
+		hipLaunchKernelGGL((k_rot_fused_pipe<TB, group_depth(TB)>),
+			dim3((uint32_t)grid), dim3(kBlock), 0, g_stream, a, P,
+			ntiles);
+	} else {
+		hipLaunchKernelGGL((k_rot_fused<TB, group_depth(TB)>),
+			dim3((uint32_t)ntiles), dim3(kBlock), 0, g_stream, a, P);
+	}
 	HIPCHK(hipGetLastError());
 	return prof_end(stop);
 }
@@ -676,6 +809,29 @@ uint32_t reduce32(const uint32_t *v, size_t n, uint32_t *u, uint32_t cap)
 	return d;
 }
 
+int g_plain_form = 2;   // unroll 2, 4 or 8; +16: non-temporal stores
+
+template <int UNR, bool NT>
+int plain_one(double2 *a, uint32_t nqb, const struct ph2hip_rot &r)
+{
+	const uint64_t namp = UINT64_C(1) << nqb;
+	const uint64_t x = r.x, z = r.z;
+	if (x == 0) {
+		hipLaunchKernelGGL((k_rot_diag<UNR, NT>),
+			dim3(grid_for(namp, kBlock * UNR, 0x7fffffff)), dim3(kBlock),
+			0, g_stream, a, namp, z, r.c, r.s);
+	} else {
+		const uint64_t npairs = namp / 2;
+		const uint32_t y = __builtin_popcountll(x & z) & 3;
+		hipLaunchKernelGGL((k_rot_pair<UNR, NT>),
+			dim3(grid_for(npairs, kBlock * UNR, 0x7fffffff)),
+			dim3(kBlock), 0, g_stream, a, npairs, x, z,
+			(uint32_t)top_bit(x), y, r.c, r.s);
+	}
+	HIPCHK(hipGetLastError());
+	return 0;
+}
+
 } // namespace
 
 // ---------------------------------------------------------------------------
@@ -699,6 +855,13 @@ int ph2hip_init(int device)
 	HIPCHK(hipSetDevice(device));
 	HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
 	g_device = device;
+	int ncu = 0;
+	if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+		    device) == hipSuccess && ncu > 0)
+		g_persist_wgs = 2 * (uint64_t)ncu;
+	const char *pw = getenv("PHASE2_HIP_FUSED_WGS");
+	if (pw && atoll(pw) > 0)
+		g_persist_wgs = (uint64_t)atoll(pw);
 	return 0;
 }
 
@@ -801,34 +964,36 @@ int ph2hip_rotate_plain(void *amp, uint32_t nqb, const struct ph2hip_rot *rot,
 		return -1;
 	if (nqb == 0 || nqb > 40)
 		return fail("rotate_plain: nqb=%u out of range", nqb);
-	const uint64_t namp = UINT64_C(1) << nqb;
 	double2 *a = (double2 *)amp;
-	constexpr int UNR = 2;
 	for (size_t k = 0; k < n; k++) {
 		const uint64_t x = rot[k].x, z = rot[k].z;
 		if (x >> nqb)
 			return fail("rotate_plain: term %zu flips qubit >= %u", k,
 				nqb);
 		hipEvent_t stop;
-		if (prof_begin(x == 0 ? 1 : 0, 1, &stop))
+		ph2hip_launch_rec meta{};
+		meta.nqb = nqb;
+		meta.xunion = x;
+		meta.zunion = z;
+		if (prof_begin(x == 0 ? 1 : 0, 1, &stop, &meta))
 			return -1;
-		if (x == 0) {
-			hipLaunchKernelGGL(k_rot_diag<UNR>,
-				dim3(grid_for(namp, kBlock * UNR, 0x7fffffff)),
-				dim3(kBlock), 0, g_stream, a, namp, z, rot[k].c,
-				rot[k].s);
-		} else {
-			const uint64_t npairs = namp / 2;
-			const uint32_t y = __builtin_popcountll(x & z) & 3;
-			hipLaunchKernelGGL(k_rot_pair<UNR>,
-				dim3(grid_for(npairs, kBlock * UNR, 0x7fffffff)),
-				dim3(kBlock), 0, g_stream, a, npairs, x, z,
-				(uint32_t)top_bit(x), y, rot[k].c, rot[k].s);
+		int rc;
+		switch (g_plain_form) {
+		case 4: rc = plain_one<4, false>(a, nqb, rot[k]); break;
+		case 8: rc = plain_one<8, false>(a, nqb, rot[k]); break;
+		case 2 + 16: rc = plain_one<2, true>(a, nqb, rot[k]); break;
+		case 4 + 16: rc = plain_one<4, true>(a, nqb, rot[k]); break;
+		default: rc = plain_one<2, false>(a, nqb, rot[k]); break;
 		}
-		HIPCHK(hipGetLastError());
-		if (prof_end(stop))
+		if (rc || prof_end(stop))
 			return -1;
 	}
+	return 0;
+}
+
+int ph2hip_set_plain_form(int form)
+{
+	g_plain_form = form;
 	return 0;
 }
 
@@ -902,6 +1067,15 @@ int ph2hip_rotate_fused(void *amp, uint32_t nqb, const struct ph2hip_rot *rot,
 			P.w[b] = w[b];
 			P.piv[b] = (uint8_t)top_bit(w[b]);
 		}
+		// row part of element tid + m*kBlock that is uniform in m
+		for (uint32_t m = 0; m < (1u << tb) / kBlock; m++) {
+			const uint32_t rm = (m * kBlock) >> L;
+			uint64_t wm = 0;
+			for (uint32_t b = 0; b < g; b++)
+				if (rm >> b & 1)
+					wm ^= w[b];
+			P.roff[m] = wm;
+		}
 		const uint64_t lm = (UINT64_C(1) << L) - 1;
 		uint32_t xt[PH2HIP_PASS_MAX];
 		for (size_t k = 0; k < len; k++) {
@@ -966,6 +1140,31 @@ int ph2hip_rotate_fused(void *amp, uint32_t nqb, const struct ph2hip_rot *rot,
 				G.u[i] = (uint16_t)u[i];
 				G.piv[i] = (uint8_t)(31 - __builtin_clz(u[i]));
 			}
+			// Coset index bits -> free tile bits.  The first four (lanes
+			// 0..15 of a wave) take free bits whose LDS-slot images
+			// (tile bit t < 12 lands on slot bit t mod 4) are distinct,
+			// so a quarter-wave reads 16 different bank groups.
+			{
+				uint32_t used = 0, npos = 0;
+				for (uint32_t i = 0; i < D; i++)
+					used |= 1u << G.piv[i];
+				for (uint32_t res = 0; res < 4; res++)
+					for (uint32_t t = res; t < tb && t < 12; t += 4)
+						if (!(used >> t & 1)) {
+							used |= 1u << t;
+							G.jpos[npos++] = (uint8_t)t;
+							break;
+						}
+				for (uint32_t t = 0; t < tb; t++)
+					if (!(used >> t & 1)) {
+						used |= 1u << t;
+						G.jpos[npos++] = (uint8_t)t;
+					}
+				if (npos != tb - D) {
+					free(passes);
+					return fail("rotate_fused: coset map");
+				}
+			}
 			G.first = (uint8_t)k;
 			G.count = (uint8_t)(e - k);
 			for (size_t q = k; q < e; q++) {
@@ -992,13 +1191,21 @@ int ph2hip_rotate_fused(void *amp, uint32_t nqb, const struct ph2hip_rot *rot,
 	}
 	double2 *a = (double2 *)amp;
 	int rc = 0;
+	k0 = 0;
 	for (size_t p = 0; p < npass && rc == 0; p++) {
+		uint64_t xu = 0, zu = 0;
+		for (size_t k = k0; k < k0 + pass_len[p]; k++) {
+			xu |= rot[k].x;
+			zu |= rot[k].z;
+		}
+		k0 += pass_len[p];
+		const FusedPass &P = passes[p];
 		switch (tile_bits[p]) {
-		case 9: rc = launch_fused<9>(a, nqb, passes[p]); break;
-		case 10: rc = launch_fused<10>(a, nqb, passes[p]); break;
-		case 11: rc = launch_fused<11>(a, nqb, passes[p]); break;
-		case 12: rc = launch_fused<12>(a, nqb, passes[p]); break;
-		case 13: rc = launch_fused<13>(a, nqb, passes[p]); break;
+		case 9: rc = launch_fused<9>(a, nqb, P, xu, zu); break;
+		case 10: rc = launch_fused<10>(a, nqb, P, xu, zu); break;
+		case 11: rc = launch_fused<11>(a, nqb, P, xu, zu); break;
+		case 12: rc = launch_fused<12>(a, nqb, P, xu, zu); break;
+		case 13: rc = launch_fused<13>(a, nqb, P, xu, zu); break;
 		default: rc = fail("rotate_fused: tile_bits"); break;
 		}
 	}
@@ -1183,6 +1390,31 @@ int ph2hip_prof_read(struct ph2hip_prof *out, int reset)
 	*out = g_pacc;
 	if (reset)
 		g_pacc = ph2hip_prof{};
+	return 0;
+}
+
+long ph2hip_prof_records(struct ph2hip_launch_rec *out, size_t cap,
+	int reset)
+{
+	if (ensure_init() || prof_drain())
+		return -1;
+	size_t n = g_rec_n < cap ? g_rec_n : cap;
+	const size_t skip = g_rec_n - n;   // keep the newest
+	for (size_t i = 0; i < n; i++)
+		out[i] = g_rec[(g_rec_head + skip + i) % PH2HIP_REC_MAX];
+	if (reset)
+		g_rec_n = g_rec_head = 0;
+	return (long)n;
+}
+
+int ph2hip_set_fused_form(uint32_t pipe_mask, uint64_t persistent_wgs)
+{
+	if (ensure_init())
+		return -1;
+	g_pipe_mask = pipe_mask;
+	g_pipe_init = true;
+	if (persistent_wgs)
+		g_persist_wgs = persistent_wgs;
 	return 0;
 }
 

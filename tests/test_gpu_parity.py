@@ -271,3 +271,29 @@ def test_fused_equals_plain_n24():
     assert abs(np.linalg.norm(a) - 1.0) < 1e-12
     back = nat.rotate(a, xm[::-1], zm[::-1], -phi[::-1], nat.FUSED, 12, 6)
     assert np.max(np.abs(back - st)) < 1e-12
+
+
+@pytest.mark.parametrize("pipe_tile", [10, 11, 12])
+def test_fused_pipelined_form_bit_identical(oracle, pipe_tile):
+    """The persistent pipelined fused kernel and every plain-kernel variant
+    leave the state bit-identical to the oracle."""
+    n = 18
+    rng = np.random.default_rng(40 + pipe_tile)
+    st = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    hm = synth.hamiltonian(n, 900, seed=pipe_tile, decay=True)
+    perm = oracle.sort_lex(hm.xm, hm.zm)
+    xm, zm = hm.xm[perm], hm.zm[perm]
+    phi = 0.3 * hm.cf[perm] / np.max(np.abs(hm.cf))
+    want = oracle.rotate(st, xm, zm, phi)
+    try:
+        nat.set_fused_form(1 << pipe_tile, 64)
+        got = nat.rotate(st, xm, zm, phi, nat.FUSED, pipe_tile, 4)
+        assert np.array_equal(got, want)
+        for form in (4, 8, 2 + 16, 4 + 16):
+            nat.set_plain_form(form)
+            got = nat.rotate(st, xm[:40], zm[:40], phi[:40], nat.PLAIN)
+            assert np.array_equal(got, oracle.rotate(st, xm[:40], zm[:40],
+                                                     phi[:40])), form
+    finally:
+        nat.set_fused_form(0, 0)
+        nat.set_plain_form(2)
