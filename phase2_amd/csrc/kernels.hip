@@ -124,6 +124,9 @@ __device__ __forceinline__ void rot_diag(double2 &a, int odd_parity,
 }
 
 constexpr int kBlock = 256;
+// HIP limits gridDim.x * blockDim.x to below 2^32; rotation kernels loop
+// over the work with at most this many workgroups.
+constexpr uint32_t kMaxGrid = 1u << 22;
 
 // ---------------------------------------------------------------------------
 // plain pass
@@ -147,7 +150,9 @@ __global__ __launch_bounds__(kBlock) void k_rot_pair(
 	double2 *__restrict__ a, uint64_t npairs, uint64_t x, uint64_t z,
 	uint32_t h, uint32_t y, double c, double s)
 {
-	const uint64_t t0 = (uint64_t)blockIdx.x * (kBlock * UNR) + threadIdx.x;
+	for (uint64_t chunk = blockIdx.x; chunk * (kBlock * UNR) < npairs;
+		chunk += gridDim.x) {
+	const uint64_t t0 = chunk * (kBlock * UNR) + threadIdx.x;
 	double2 vi[UNR], vj[UNR];
 	uint64_t ii[UNR];
 #pragma unroll
@@ -169,13 +174,16 @@ __global__ __launch_bounds__(kBlock) void k_rot_pair(
 			st16<NT>(&a[ii[u] ^ x], vj[u]);
 		}
 	}
+	}
 }
 
 template <int UNR, bool NT>
 __global__ __launch_bounds__(kBlock) void k_rot_diag(
 	double2 *__restrict__ a, uint64_t namp, uint64_t z, double c, double s)
 {
-	const uint64_t i0 = (uint64_t)blockIdx.x * (kBlock * UNR) + threadIdx.x;
+	for (uint64_t chunk = blockIdx.x; chunk * (kBlock * UNR) < namp;
+		chunk += gridDim.x) {
+	const uint64_t i0 = chunk * (kBlock * UNR) + threadIdx.x;
 	double2 v[UNR];
 #pragma unroll
 	for (int u = 0; u < UNR; u++) {
@@ -190,6 +198,7 @@ __global__ __launch_bounds__(kBlock) void k_rot_diag(
 			rot_diag(v[u], __popcll(i & z) & 1, c, s);
 			st16<NT>(&a[i], v[u]);
 		}
+	}
 	}
 }
 
@@ -404,7 +413,7 @@ __device__ __forceinline__ void tile_groups(v2d *tile, const FusedPass &P,
 // by the co-resident workgroups of the CU.
 template <int TB, int D>
 __global__ __launch_bounds__(kBlock) void k_rot_fused(
-	double2 *__restrict__ amp, const FusedPass P)
+	double2 *__restrict__ amp, const FusedPass P, uint64_t ntiles)
 {
 	constexpr uint32_t PER = (1u << TB) / kBlock;
 	__shared__ v2d tile[1u << TB];
@@ -415,7 +424,11 @@ __global__ __launch_bounds__(kBlock) void k_rot_fused(
 	for (uint32_t b = 0; b < P.g; b++)
 		if ((tid >> L) >> b & 1)
 			wt ^= P.w[b];
-	const uint64_t base = tile_base(P, blockIdx.x);
+	// 2-D grid when the tiles outnumber kMaxGrid (gridDim.x * 256 < 2^32)
+	const uint64_t t = (uint64_t)blockIdx.y * gridDim.x + blockIdx.x;
+	if (t >= ntiles)
+		return;           // whole workgroup: no barrier is left waiting
+	const uint64_t base = tile_base(P, t);
 	const uint64_t bo = base ^ wt;
 #pragma unroll 4
 	for (uint32_t m = 0; m < PER; m++)
@@ -491,8 +504,8 @@ __global__ __launch_bounds__(kBlock) void k_gather(
 	const double2 *__restrict__ a, const uint64_t *__restrict__ idx,
 	double2 *__restrict__ out, uint64_t n)
 {
-	const uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-	if (k < n)
+	for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < n;
+		k += (uint64_t)gridDim.x * kBlock)
 		out[k] = a[idx[k]];
 }
 
@@ -779,8 +792,11 @@ int launch_fused(double2 *a, uint32_t nqb, const FusedPass &P,
 			dim3((uint32_t)grid), dim3(kBlock), 0, g_stream, a, P,
 			ntiles);
 	} else {
+		const uint64_t gx = ntiles < kMaxGrid ? ntiles : kMaxGrid;
+		const uint64_t gy = (ntiles + gx - 1) / gx;
 		hipLaunchKernelGGL((k_rot_fused<TB, group_depth(TB)>),
-			dim3((uint32_t)ntiles), dim3(kBlock), 0, g_stream, a, P);
+			dim3((uint32_t)gx, (uint32_t)gy), dim3(kBlock), 0, g_stream,
+			a, P, ntiles);
 	}
 	HIPCHK(hipGetLastError());
 	return prof_end(stop);
@@ -818,13 +834,13 @@ int plain_one(double2 *a, uint32_t nqb, const struct ph2hip_rot &r)
 	const uint64_t x = r.x, z = r.z;
 	if (x == 0) {
 		hipLaunchKernelGGL((k_rot_diag<UNR, NT>),
-			dim3(grid_for(namp, kBlock * UNR, 0x7fffffff)), dim3(kBlock),
+			dim3(grid_for(namp, kBlock * UNR, kMaxGrid)), dim3(kBlock),
 			0, g_stream, a, namp, z, r.c, r.s);
 	} else {
 		const uint64_t npairs = namp / 2;
 		const uint32_t y = __builtin_popcountll(x & z) & 3;
 		hipLaunchKernelGGL((k_rot_pair<UNR, NT>),
-			dim3(grid_for(npairs, kBlock * UNR, 0x7fffffff)),
+			dim3(grid_for(npairs, kBlock * UNR, kMaxGrid)),
 			dim3(kBlock), 0, g_stream, a, npairs, x, z,
 			(uint32_t)top_bit(x), y, r.c, r.s);
 	}
@@ -1237,7 +1253,7 @@ int ph2hip_gather(const void *amp, uint32_t nqb, const uint64_t *idx,
 		    hipMemcpyHostToDevice, g_stream) != hipSuccess)
 		rc = fail("gather: copy in");
 	if (!rc) {
-		hipLaunchKernelGGL(k_gather, dim3(grid_for(n, kBlock, 1u << 30)),
+		hipLaunchKernelGGL(k_gather, dim3(grid_for(n, kBlock, kMaxGrid)),
 			dim3(kBlock), 0, g_stream, (const double2 *)amp, didx,
 			dout, (uint64_t)n);
 		if (hipGetLastError() != hipSuccess)
