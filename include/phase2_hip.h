@@ -95,6 +95,41 @@ int ph2hip_set_fused_form(uint32_t pipe_mask, uint64_t persistent_wgs);
 uint32_t ph2hip_span_rank(const struct ph2hip_rot *rot, size_t n,
 	uint32_t low_bits);
 
+/* -- batched registers ------------------------------------------------------ */
+
+/* One register inside a shared device buffer: 2^nqb amplitudes starting at
+ * amplitude amp_off, its rotations in order, and their fused schedule
+ * (npass passes of pass_len[p] terms, all in tiles of the call's size). */
+struct ph2hip_batch_job {
+	uint64_t amp_off;
+	uint32_t nqb;
+	const struct ph2hip_rot *rot;
+	size_t n;
+	const uint32_t *pass_len;
+	size_t npass;
+};
+
+/* Apply every job's rotations, bit-identical to ph2hip_rotate_plain on each
+ * register alone.  Launch p applies pass p of every job that has one, so a
+ * batch costs max(npass) launches instead of sum(n); with use_graph the
+ * launches are captured in one hipGraph and replayed as a single submission
+ * (not while profiling).  *launches (if non-NULL) gets the launch count.
+ * Returns when the work is complete.  No counterpart in the reference
+ * (which runs samples one after another, circ/qdrift.c:115-131). */
+int ph2hip_rotate_batched(void *amp, uint64_t namp_total,
+	const struct ph2hip_batch_job *jobs, size_t njobs, uint32_t tile_bits,
+	uint32_t low_bits, int use_graph, uint64_t *launches);
+
+/* amp[idx[k]] = (val[2k], val[2k+1]) with idx < namp_total (host arrays);
+ * for repeated indices the last entry wins, as sequential qreg_setamp
+ * calls would leave it. */
+int ph2hip_scatter_set(void *amp, uint64_t namp_total, const uint64_t *idx,
+	const double *val, size_t n);
+
+/* out[2k], out[2k+1] = amp[idx[k]], idx < namp_total. */
+int ph2hip_gather_abs(const void *amp, uint64_t namp_total,
+	const uint64_t *idx, size_t n, double *out);
+
 /* -- overlap and Slater-Condon -------------------------------------------- */
 
 /* out[2k], out[2k+1] = amp[idx[k]] (host arrays; idx < 2^nqb checked).
@@ -128,8 +163,9 @@ int ph2hip_sc_inner(const void *amp, uint32_t nqb, const struct ph2hip_sc *sc,
  * ph2hip_rotate_plain / ph2hip_rotate_fused is bracketed by HIP events on
  * the library stream; ph2hip_prof_read synchronises and returns launch
  * counts and summed kernel milliseconds per kind since the last reset.
- * Kinds: 0 plain pair pass, 1 plain diagonal pass, 2 fused pass. */
-#define PH2HIP_PROF_KINDS 3
+ * Kinds: 0 plain pair pass, 1 plain diagonal pass, 2 fused pass, 3 batched
+ * fused pass. */
+#define PH2HIP_PROF_KINDS 4
 struct ph2hip_prof {
 	uint64_t launches[PH2HIP_PROF_KINDS];
 	double ms[PH2HIP_PROF_KINDS];

@@ -238,9 +238,10 @@ def sort_lex(xm, zm):
 
 
 class Prof(C.Structure):
-    """struct ph2hip_prof: kinds 0 plain pair, 1 plain diagonal, 2 fused."""
-    _fields_ = [("launches", C.c_uint64 * 3), ("ms", C.c_double * 3),
-                ("terms", C.c_uint64 * 3)]
+    """struct ph2hip_prof: kinds 0 plain pair, 1 plain diagonal, 2 fused,
+    3 batched fused."""
+    _fields_ = [("launches", C.c_uint64 * 4), ("ms", C.c_double * 4),
+                ("terms", C.c_uint64 * 4)]
 
 
 def prof_enable(on: bool):
@@ -252,7 +253,7 @@ def prof_read(reset=True) -> dict:
     p = Prof()
     if lib().ph2hip_prof_read(C.byref(p), C.c_int(int(reset))) != 0:
         _err("ph2hip_prof_read")
-    kinds = ("plain_pair", "plain_diag", "fused")
+    kinds = ("plain_pair", "plain_diag", "fused", "batched")
     return {k: dict(launches=int(p.launches[i]), ms=float(p.ms[i]),
                     terms=int(p.terms[i])) for i, k in enumerate(kinds)}
 
@@ -362,3 +363,59 @@ def set_fused_form(pipe_mask: int, persistent_wgs: int = 0):
 def set_plain_form(form: int):
     """2, 4 or 8 pairs per thread; +16 for non-temporal stores."""
     lib().ph2hip_set_plain_form(C.c_int(form))
+
+
+# -- batched qDRIFT ------------------------------------------------------------
+
+class BatchReg(C.Structure):
+    """struct ph2_batch_reg (include/phase2_batch.h)."""
+    _fields_ = [("nqb", C.c_uint32), ("nterms", C.c_size_t),
+                ("xm", _u64p), ("zm", _u64p), ("cf", _f64p),
+                ("ndet", C.c_size_t), ("didx", _u64p), ("dcf", _f64p),
+                ("samples", C.c_size_t), ("depth", C.c_size_t),
+                ("step_size", C.c_double), ("seed", C.c_uint64)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("seconds", C.c_double), ("tune_seconds", C.c_double),
+                ("amp_updates", C.c_double), ("launches", C.c_uint64),
+                ("passes", C.c_uint64), ("tile_bits", C.c_uint32),
+                ("low_bits", C.c_uint32)]
+
+
+BATCH_PLAIN_EACH, BATCH_FUSED_EACH, BATCH_FUSED_BATCHED = 0, 1, 2
+
+
+def batch_qdrift(regs, mode=BATCH_FUSED_BATCHED, tile_bits=0, low_bits=4):
+    """regs: list of dicts with hm, det_idx, det_cf, samples, depth,
+    step_size, seed.  Returns (list of per-register overlap arrays, stats)."""
+    keep = []
+    arr = (BatchReg * len(regs))()
+    total = 0
+    for i, r in enumerate(regs):
+        hm = r["hm"]
+        xm, zm, cf = _terms(hm)
+        idx = np.ascontiguousarray(r["det_idx"], dtype=np.uint64)
+        dcf = np.ascontiguousarray(
+            np.asarray(r["det_cf"], dtype=np.complex128).view(np.float64))
+        keep += [xm, zm, cf, idx, dcf]
+        arr[i] = BatchReg(hm.nqb, len(cf), _p(xm, _u64p), _p(zm, _u64p),
+                          _p(cf, _f64p), idx.shape[0], _p(idx, _u64p),
+                          _p(dcf, _f64p), r["samples"], r["depth"],
+                          r["step_size"], r["seed"])
+        total += r["samples"]
+    out = np.zeros(max(total, 1), dtype=np.complex128)
+    st = BatchStats()
+    L = lib()
+    L.ph2_batch_qdrift.restype = C.c_int
+    rc = L.ph2_batch_qdrift(arr, C.c_size_t(len(regs)), C.c_int(mode),
+                            C.c_uint32(tile_bits), C.c_uint32(low_bits),
+                            _p(out, _f64p), C.byref(st))
+    if rc != 0:
+        _err("ph2_batch_qdrift")
+    res, o = [], 0
+    for r in regs:
+        res.append(out[o:o + r["samples"]].copy())
+        o += r["samples"]
+    stats = {f: getattr(st, f) for f, _ in BatchStats._fields_}
+    return res, stats

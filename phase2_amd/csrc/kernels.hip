@@ -29,6 +29,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <cstdarg>
+#include <algorithm>
 
 #include "phase2_hip.h"
 
@@ -409,25 +410,17 @@ __device__ __forceinline__ void tile_groups(v2d *tile, const FusedPass &P,
 	__syncthreads();
 }
 
-// One tile per workgroup (grid = number of tiles); HBM latency is hidden
-// by the co-resident workgroups of the CU.
+// Load one tile from HBM into LDS, apply the pass, write it back.
 template <int TB, int D>
-__global__ __launch_bounds__(kBlock) void k_rot_fused(
-	double2 *__restrict__ amp, const FusedPass P, uint64_t ntiles)
+__device__ __forceinline__ void fused_tile(v2d *__restrict__ a,
+	v2d *tile, const FusedPass &P, uint64_t t)
 {
 	constexpr uint32_t PER = (1u << TB) / kBlock;
-	__shared__ v2d tile[1u << TB];
-	// clang vector type: plain 16-byte accesses that stay in registers
-	v2d *__restrict__ a = reinterpret_cast<v2d *>(amp);
 	const uint32_t L = P.L, lmask = (1u << L) - 1, tid = threadIdx.x;
 	uint64_t wt = 0;   // gathered offset of this thread's row part
 	for (uint32_t b = 0; b < P.g; b++)
 		if ((tid >> L) >> b & 1)
 			wt ^= P.w[b];
-	// 2-D grid when the tiles outnumber kMaxGrid (gridDim.x * 256 < 2^32)
-	const uint64_t t = (uint64_t)blockIdx.y * gridDim.x + blockIdx.x;
-	if (t >= ntiles)
-		return;           // whole workgroup: no barrier is left waiting
 	const uint64_t base = tile_base(P, t);
 	const uint64_t bo = base ^ wt;
 #pragma unroll 4
@@ -439,6 +432,52 @@ __global__ __launch_bounds__(kBlock) void k_rot_fused(
 	for (uint32_t m = 0; m < PER; m++)
 		a[bo ^ P.roff[m] ^ ((tid + m * kBlock) & lmask)] =
 			tile[lds_slot(tid + m * kBlock)];
+}
+
+// One tile per workgroup (grid = number of tiles); HBM latency is hidden
+// by the co-resident workgroups of the CU.
+template <int TB, int D>
+__global__ __launch_bounds__(kBlock) void k_rot_fused(
+	double2 *__restrict__ amp, const FusedPass P, uint64_t ntiles)
+{
+	__shared__ v2d tile[1u << TB];
+	// 2-D grid when the tiles outnumber kMaxGrid (gridDim.x * 256 < 2^32)
+	const uint64_t t = (uint64_t)blockIdx.y * gridDim.x + blockIdx.x;
+	if (t >= ntiles)
+		return;           // whole workgroup: no barrier is left waiting
+	// clang vector type: plain 16-byte accesses that stay in registers
+	fused_tile<TB, D>(reinterpret_cast<v2d *>(amp), tile, P, t);
+}
+
+// Batched pass: pass p of many registers in one launch.  Work item i covers
+// tiles [tile0_i, tile0_{i+1}) of register i (amplitudes at amp_off_i) with
+// descriptor desc_i; a workgroup finds its item by a uniform binary search.
+struct BatchItem {
+	uint64_t amp_off;
+	uint64_t tile0;
+	uint64_t desc;
+};
+
+template <int TB, int D>
+__global__ __launch_bounds__(kBlock) void k_rot_fused_batch(
+	double2 *__restrict__ amp, const FusedPass *__restrict__ descs,
+	const BatchItem *__restrict__ items, uint32_t nitems, uint64_t ntiles)
+{
+	__shared__ v2d tile[1u << TB];
+	const uint64_t t = (uint64_t)blockIdx.y * gridDim.x + blockIdx.x;
+	if (t >= ntiles)
+		return;
+	uint32_t lo = 0, hi = nitems;
+	while (hi - lo > 1) {
+		const uint32_t mid = (lo + hi) / 2;
+		if (items[mid].tile0 <= t)
+			lo = mid;
+		else
+			hi = mid;
+	}
+	const BatchItem it = items[lo];
+	fused_tile<TB, D>(reinterpret_cast<v2d *>(amp) + it.amp_off, tile,
+		descs[it.desc], t - it.tile0);
 }
 
 // Persistent, software-pipelined form: each workgroup walks tiles
@@ -507,6 +546,15 @@ __global__ __launch_bounds__(kBlock) void k_gather(
 	for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < n;
 		k += (uint64_t)gridDim.x * kBlock)
 		out[k] = a[idx[k]];
+}
+
+__global__ __launch_bounds__(kBlock) void k_scatter_set(
+	double2 *__restrict__ a, const uint64_t *__restrict__ idx,
+	const double2 *__restrict__ val, uint64_t n)
+{
+	for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < n;
+		k += (uint64_t)gridDim.x * kBlock)
+		a[idx[k]] = val[k];
 }
 
 __device__ __forceinline__ uint64_t taper_idx(uint64_t v, uint32_t n_sites)
@@ -848,6 +896,186 @@ int plain_one(double2 *a, uint32_t nqb, const struct ph2hip_rot &r)
 	return 0;
 }
 
+// Descriptor of one fused pass over terms r[0..len) (global term index k0
+// for messages) in tiles of 2^tb amplitudes.  0, or -1 if the pass does not
+// fit its tile.
+int build_pass(FusedPass &P, const struct ph2hip_rot *r, size_t len,
+	uint32_t tb, uint32_t low_bits, size_t k0)
+{
+	memset(&P, 0, sizeof P);
+	// Grow the contiguous low part L while the gathered rank g fits:
+	// L = tb - g(L) at the fixed point (g is non-increasing in L).
+	uint32_t L = low_bits < tb ? low_bits : tb;
+	uint32_t g = 0;
+	uint64_t w[PH2HIP_TILE_MAX + 1];
+	for (;;) {
+		const uint64_t lm = (UINT64_C(1) << L) - 1;
+		g = reduce_basis(r, len, lm, w, tb);
+		if (L + g > tb) {
+			return fail("rotate_fused: pass at term %zu needs %u+%u > %u "
+			    "tile bits", k0, L, g, tb);
+		}
+		if (L + g == tb)
+			break;
+		L = tb - g;
+	}
+	P.L = (uint8_t)L;
+	P.g = (uint8_t)g;
+	P.nterms = (uint8_t)len;
+	for (uint32_t b = 0; b < g; b++) {
+		P.w[b] = w[b];
+		P.piv[b] = (uint8_t)top_bit(w[b]);
+	}
+	// row part of element tid + m*kBlock that is uniform in m
+	for (uint32_t m = 0; m < (1u << tb) / kBlock; m++) {
+		const uint32_t rm = (m * kBlock) >> L;
+		uint64_t wm = 0;
+		for (uint32_t b = 0; b < g; b++)
+			if (rm >> b & 1)
+				wm ^= w[b];
+		P.roff[m] = wm;
+	}
+	const uint64_t lm = (UINT64_C(1) << L) - 1;
+	uint32_t xt[PH2HIP_PASS_MAX];
+	for (size_t k = 0; k < len; k++) {
+		const uint64_t x = r[k].x, z = r[k].z;
+		uint32_t xk = (uint32_t)(x & lm), zk = (uint32_t)(z & lm);
+		uint64_t rest = x & ~lm;
+		for (uint32_t b = 0; b < g; b++) {
+			if (x >> P.piv[b] & 1) {
+				xk |= 1u << (L + b);
+				rest ^= w[b];
+			}
+			if (__builtin_popcountll(w[b] & z) & 1)
+				zk |= 1u << (L + b);
+		}
+		if (rest) {
+			return fail("rotate_fused: term %zu outside its tile",
+				k0 + k);
+		}
+		xt[k] = xk;
+		FusedTerm &T = P.t[k];
+		T.z = z;
+		T.zt = zk;
+		T.c = r[k].c;
+		T.s = r[k].s;
+		T.y = (uint8_t)(__builtin_popcountll(x & z) & 3);
+	}
+	// micro-groups: consecutive terms whose xt span <= D dims
+	const uint32_t D = (uint32_t)group_depth((int)tb);
+	uint32_t ng = 0;
+	size_t k = 0;
+	while (k < len) {
+		size_t e = k + 1;
+		uint32_t u[8];
+		while (e < len && reduce32(xt + k, e + 1 - k, u, D) <= D)
+			e++;
+		uint32_t d = reduce32(xt + k, e - k, u, D);
+		// pad with unit vectors of tile bits outside the span
+		for (uint32_t bit = 0; d < D && bit < tb; bit++) {
+			uint32_t x = 1u << bit;
+			for (uint32_t i = 0; i < d && x; i++)
+				if (x >> (31 - __builtin_clz(u[i])) & 1)
+					x ^= u[i];
+			if (!x)
+				continue;
+			const int p = 31 - __builtin_clz(x);
+			for (uint32_t i = 0; i < d; i++)
+				if (u[i] >> p & 1)
+					u[i] ^= x;
+			u[d++] = x;
+		}
+		for (uint32_t i = 1; i < d; i++)   // ascending pivots
+			for (uint32_t j = i; j > 0 && __builtin_clz(u[j - 1]) <
+				__builtin_clz(u[j]); j--) {
+				const uint32_t t = u[j];
+				u[j] = u[j - 1];
+				u[j - 1] = t;
+			}
+		FusedGroup &G = P.grp[ng++];
+		memset(&G, 0, sizeof G);
+		for (uint32_t i = 0; i < D; i++) {
+			G.u[i] = (uint16_t)u[i];
+			G.piv[i] = (uint8_t)(31 - __builtin_clz(u[i]));
+		}
+		// Coset index bits -> free tile bits.  The first four (lanes
+		// 0..15 of a wave) take free bits whose LDS-slot images
+		// (tile bit t < 12 lands on slot bit t mod 4) are distinct,
+		// so a quarter-wave reads 16 different bank groups.
+		{
+			uint32_t used = 0, npos = 0;
+			for (uint32_t i = 0; i < D; i++)
+				used |= 1u << G.piv[i];
+			for (uint32_t res = 0; res < 4; res++)
+				for (uint32_t t = res; t < tb && t < 12; t += 4)
+					if (!(used >> t & 1)) {
+						used |= 1u << t;
+						G.jpos[npos++] = (uint8_t)t;
+						break;
+					}
+			for (uint32_t t = 0; t < tb; t++)
+				if (!(used >> t & 1)) {
+					used |= 1u << t;
+					G.jpos[npos++] = (uint8_t)t;
+				}
+			if (npos != tb - D)
+				return fail("rotate_fused: coset map");
+		}
+		G.first = (uint8_t)k;
+		G.count = (uint8_t)(e - k);
+		for (size_t q = k; q < e; q++) {
+			uint32_t xp = 0, zp = 0, rest = xt[q];
+			for (uint32_t i = 0; i < D; i++) {
+				if (xt[q] >> G.piv[i] & 1) {
+					xp |= 1u << i;
+					rest ^= u[i];
+				}
+				if (__builtin_popcount(u[i] & P.t[q].zt) & 1)
+					zp |= 1u << i;
+			}
+			if (rest)
+				return fail("rotate_fused: group basis");
+			P.t[q].xtp = (uint8_t)xp;
+			P.t[q].ztp = (uint8_t)zp;
+		}
+		k = e;
+	}
+	P.ngroups = (uint8_t)ng;
+	return 0;
+}
+
+// Grow-only device buffer.
+int dev_reserve(void **buf, size_t *cap, size_t bytes)
+{
+	if (bytes <= *cap)
+		return 0;
+	if (*buf) {
+		HIPCHK(hipStreamSynchronize(g_stream));
+		HIPCHK(hipFree(*buf));
+		*buf = nullptr;
+		*cap = 0;
+	}
+	HIPCHK(hipMalloc(buf, bytes));
+	*cap = bytes;
+	return 0;
+}
+
+void *g_bdesc = nullptr, *g_bitem = nullptr;
+size_t g_bdesc_cap = 0, g_bitem_cap = 0;
+
+template <int TB>
+int launch_batch(double2 *amp, const FusedPass *descs, const BatchItem *items,
+	uint32_t nitems, uint64_t ntiles)
+{
+	const uint64_t gx = ntiles < kMaxGrid ? ntiles : kMaxGrid;
+	const uint64_t gy = (ntiles + gx - 1) / gx;
+	hipLaunchKernelGGL((k_rot_fused_batch<TB, group_depth(TB)>),
+		dim3((uint32_t)gx, (uint32_t)gy), dim3(kBlock), 0, g_stream, amp,
+		descs, items, nitems, ntiles);
+	HIPCHK(hipGetLastError());
+	return 0;
+}
+
 } // namespace
 
 // ---------------------------------------------------------------------------
@@ -1055,155 +1283,12 @@ int ph2hip_rotate_fused(void *amp, uint32_t nqb, const struct ph2hip_rot *rot,
 		return fail("rotate_fused: out of host memory");
 	size_t k0 = 0;
 	for (size_t p = 0; p < npass; p++) {
-		FusedPass &P = passes[p];
-		const uint32_t tb = tile_bits[p];
-		const struct ph2hip_rot *r = rot + k0;
-		const size_t len = pass_len[p];
-		// Grow the contiguous low part L while the gathered rank g fits:
-		// L = tb - g(L) at the fixed point (g is non-increasing in L).
-		uint32_t L = low_bits < tb ? low_bits : tb;
-		uint32_t g = 0;
-		uint64_t w[PH2HIP_TILE_MAX + 1];
-		for (;;) {
-			const uint64_t lm = (UINT64_C(1) << L) - 1;
-			g = reduce_basis(r, len, lm, w, tb);
-			if (L + g > tb) {
-				free(passes);
-				return fail("rotate_fused: pass %zu needs %u+%u > %u "
-					    "tile bits", p, L, g, tb);
-			}
-			if (L + g == tb)
-				break;
-			L = tb - g;
+		if (build_pass(passes[p], rot + k0, pass_len[p], tile_bits[p],
+			    low_bits, k0)) {
+			free(passes);
+			return -1;
 		}
-		P.L = (uint8_t)L;
-		P.g = (uint8_t)g;
-		P.nterms = (uint8_t)len;
-		for (uint32_t b = 0; b < g; b++) {
-			P.w[b] = w[b];
-			P.piv[b] = (uint8_t)top_bit(w[b]);
-		}
-		// row part of element tid + m*kBlock that is uniform in m
-		for (uint32_t m = 0; m < (1u << tb) / kBlock; m++) {
-			const uint32_t rm = (m * kBlock) >> L;
-			uint64_t wm = 0;
-			for (uint32_t b = 0; b < g; b++)
-				if (rm >> b & 1)
-					wm ^= w[b];
-			P.roff[m] = wm;
-		}
-		const uint64_t lm = (UINT64_C(1) << L) - 1;
-		uint32_t xt[PH2HIP_PASS_MAX];
-		for (size_t k = 0; k < len; k++) {
-			const uint64_t x = r[k].x, z = r[k].z;
-			uint32_t xk = (uint32_t)(x & lm), zk = (uint32_t)(z & lm);
-			uint64_t rest = x & ~lm;
-			for (uint32_t b = 0; b < g; b++) {
-				if (x >> P.piv[b] & 1) {
-					xk |= 1u << (L + b);
-					rest ^= w[b];
-				}
-				if (__builtin_popcountll(w[b] & z) & 1)
-					zk |= 1u << (L + b);
-			}
-			if (rest) {
-				free(passes);
-				return fail("rotate_fused: term %zu outside its tile",
-					k0 + k);
-			}
-			xt[k] = xk;
-			FusedTerm &T = P.t[k];
-			T.z = z;
-			T.zt = zk;
-			T.c = r[k].c;
-			T.s = r[k].s;
-			T.y = (uint8_t)(__builtin_popcountll(x & z) & 3);
-		}
-		// micro-groups: consecutive terms whose xt span <= D dims
-		const uint32_t D = (uint32_t)group_depth((int)tb);
-		uint32_t ng = 0;
-		size_t k = 0;
-		while (k < len) {
-			size_t e = k + 1;
-			uint32_t u[8];
-			while (e < len && reduce32(xt + k, e + 1 - k, u, D) <= D)
-				e++;
-			uint32_t d = reduce32(xt + k, e - k, u, D);
-			// pad with unit vectors of tile bits outside the span
-			for (uint32_t bit = 0; d < D && bit < tb; bit++) {
-				uint32_t x = 1u << bit;
-				for (uint32_t i = 0; i < d && x; i++)
-					if (x >> (31 - __builtin_clz(u[i])) & 1)
-						x ^= u[i];
-				if (!x)
-					continue;
-				const int p = 31 - __builtin_clz(x);
-				for (uint32_t i = 0; i < d; i++)
-					if (u[i] >> p & 1)
-						u[i] ^= x;
-				u[d++] = x;
-			}
-			for (uint32_t i = 1; i < d; i++)   // ascending pivots
-				for (uint32_t j = i; j > 0 && __builtin_clz(u[j - 1]) <
-					__builtin_clz(u[j]); j--) {
-					const uint32_t t = u[j];
-					u[j] = u[j - 1];
-					u[j - 1] = t;
-				}
-			FusedGroup &G = P.grp[ng++];
-			memset(&G, 0, sizeof G);
-			for (uint32_t i = 0; i < D; i++) {
-				G.u[i] = (uint16_t)u[i];
-				G.piv[i] = (uint8_t)(31 - __builtin_clz(u[i]));
-			}
-			// Coset index bits -> free tile bits.  The first four (lanes
-			// 0..15 of a wave) take free bits whose LDS-slot images
-			// (tile bit t < 12 lands on slot bit t mod 4) are distinct,
-			// so a quarter-wave reads 16 different bank groups.
-			{
-				uint32_t used = 0, npos = 0;
-				for (uint32_t i = 0; i < D; i++)
-					used |= 1u << G.piv[i];
-				for (uint32_t res = 0; res < 4; res++)
-					for (uint32_t t = res; t < tb && t < 12; t += 4)
-						if (!(used >> t & 1)) {
-							used |= 1u << t;
-							G.jpos[npos++] = (uint8_t)t;
-							break;
-						}
-				for (uint32_t t = 0; t < tb; t++)
-					if (!(used >> t & 1)) {
-						used |= 1u << t;
-						G.jpos[npos++] = (uint8_t)t;
-					}
-				if (npos != tb - D) {
-					free(passes);
-					return fail("rotate_fused: coset map");
-				}
-			}
-			G.first = (uint8_t)k;
-			G.count = (uint8_t)(e - k);
-			for (size_t q = k; q < e; q++) {
-				uint32_t xp = 0, zp = 0, rest = xt[q];
-				for (uint32_t i = 0; i < D; i++) {
-					if (xt[q] >> G.piv[i] & 1) {
-						xp |= 1u << i;
-						rest ^= u[i];
-					}
-					if (__builtin_popcount(u[i] & P.t[q].zt) & 1)
-						zp |= 1u << i;
-				}
-				if (rest) {
-					free(passes);
-					return fail("rotate_fused: group basis");
-				}
-				P.t[q].xtp = (uint8_t)xp;
-				P.t[q].ztp = (uint8_t)zp;
-			}
-			k = e;
-		}
-		P.ngroups = (uint8_t)ng;
-		k0 += len;
+		k0 += pass_len[p];
 	}
 	double2 *a = (double2 *)amp;
 	int rc = 0;
@@ -1480,4 +1565,235 @@ int ph2hip_norm2(const void *amp, uint32_t nqb, double *out)
 	return 0;
 }
 
+int ph2hip_rotate_batched(void *amp, uint64_t namp_total,
+	const struct ph2hip_batch_job *jobs, size_t njobs, uint32_t tile_bits,
+	uint32_t low_bits, int use_graph, uint64_t *launches)
+{
+	if (ensure_init())
+		return -1;
+	if (tile_bits < PH2HIP_TILE_MIN || tile_bits > PH2HIP_TILE_MAX)
+		return fail("rotate_batched: tile_bits %u", tile_bits);
+	size_t ndesc = 0, maxp = 0;
+	for (size_t j = 0; j < njobs; j++) {
+		const struct ph2hip_batch_job &J = jobs[j];
+		if (J.nqb < tile_bits || J.nqb > 40 ||
+		    J.amp_off + (UINT64_C(1) << J.nqb) > namp_total)
+			return fail("rotate_batched: job %zu (nqb=%u) does not fit",
+				j, J.nqb);
+		size_t tot = 0;
+		for (size_t p = 0; p < J.npass; p++) {
+			if (J.pass_len[p] == 0 || J.pass_len[p] > PH2HIP_PASS_MAX)
+				return fail("rotate_batched: job %zu pass %zu", j, p);
+			tot += J.pass_len[p];
+		}
+		if (tot != J.n)
+			return fail("rotate_batched: job %zu passes cover %zu of %zu",
+				j, tot, J.n);
+		for (size_t k = 0; k < J.n; k++)
+			if (J.rot[k].x >> J.nqb)
+				return fail("rotate_batched: job %zu term %zu", j, k);
+		ndesc += J.npass;
+		maxp = J.npass > maxp ? J.npass : maxp;
+	}
+	if (launches)
+		*launches = 0;
+	if (ndesc == 0)
+		return 0;
+	FusedPass *desc = (FusedPass *)malloc(sizeof(FusedPass) * ndesc);
+	BatchItem *items = (BatchItem *)malloc(sizeof(BatchItem) * ndesc);
+	size_t *first = (size_t *)malloc(sizeof(size_t) * njobs);
+	size_t *lstart = (size_t *)calloc(maxp + 1, sizeof(size_t));
+	uint64_t *ltiles = (uint64_t *)calloc(maxp, sizeof(uint64_t));
+	int rc = 0;
+	if (!desc || !items || !first || !lstart || !ltiles) {
+		rc = fail("rotate_batched: out of host memory");
+		goto out;
+	}
+	// descriptors, job-major
+	{
+		size_t d = 0;
+		for (size_t j = 0; j < njobs && !rc; j++) {
+			first[j] = d;
+			size_t k0 = 0;
+			for (size_t p = 0; p < jobs[j].npass && !rc; p++) {
+				rc = build_pass(desc[d++], jobs[j].rot + k0,
+					jobs[j].pass_len[p], tile_bits, low_bits, k0);
+				k0 += jobs[j].pass_len[p];
+			}
+		}
+	}
+	if (rc)
+		goto out;
+	// work items, launch-major: launch p holds pass p of every job that has it
+	{
+		size_t it = 0;
+		for (size_t p = 0; p < maxp; p++) {
+			lstart[p] = it;
+			uint64_t t0 = 0;
+			for (size_t j = 0; j < njobs; j++) {
+				if (jobs[j].npass <= p)
+					continue;
+				items[it].amp_off = jobs[j].amp_off;
+				items[it].tile0 = t0;
+				items[it].desc = first[j] + p;
+				t0 += UINT64_C(1) << (jobs[j].nqb - tile_bits);
+				it++;
+			}
+			ltiles[p] = t0;
+		}
+		lstart[maxp] = it;
+	}
+	if (dev_reserve(&g_bdesc, &g_bdesc_cap, sizeof(FusedPass) * ndesc) ||
+	    dev_reserve(&g_bitem, &g_bitem_cap, sizeof(BatchItem) * ndesc)) {
+		rc = -1;
+		goto out;
+	}
+	if (hipMemcpyAsync(g_bdesc, desc, sizeof(FusedPass) * ndesc,
+		    hipMemcpyHostToDevice, g_stream) != hipSuccess ||
+	    hipMemcpyAsync(g_bitem, items, sizeof(BatchItem) * ndesc,
+		    hipMemcpyHostToDevice, g_stream) != hipSuccess ||
+	    hipStreamSynchronize(g_stream) != hipSuccess) {
+		rc = fail("rotate_batched: upload");
+		goto out;
+	}
+	{
+		// Profiling brackets every launch with events, which stays out of
+		// graph capture.
+		const bool graph = use_graph && !g_prof;
+		hipGraph_t gr = nullptr;
+		hipGraphExec_t ge = nullptr;
+		if (graph && hipStreamBeginCapture(g_stream,
+			    hipStreamCaptureModeThreadLocal) != hipSuccess) {
+			rc = fail("rotate_batched: begin capture");
+			goto out;
+		}
+		for (size_t p = 0; p < maxp && !rc; p++) {
+			const uint32_t ni = (uint32_t)(lstart[p + 1] - lstart[p]);
+			const BatchItem *it = (const BatchItem *)g_bitem + lstart[p];
+			const FusedPass *dd = (const FusedPass *)g_bdesc;
+			hipEvent_t stop = nullptr;
+			ph2hip_launch_rec meta{};
+			meta.tile_bits = tile_bits;
+			meta.ngroups = ni;
+			if (!graph && prof_begin(3, ni, &stop, &meta)) {
+				rc = -1;
+				break;
+			}
+			double2 *a = (double2 *)amp;
+			switch (tile_bits) {
+			case 9: rc = launch_batch<9>(a, dd, it, ni, ltiles[p]); break;
+			case 10: rc = launch_batch<10>(a, dd, it, ni, ltiles[p]); break;
+			case 11: rc = launch_batch<11>(a, dd, it, ni, ltiles[p]); break;
+			case 12: rc = launch_batch<12>(a, dd, it, ni, ltiles[p]); break;
+			default: rc = launch_batch<13>(a, dd, it, ni, ltiles[p]); break;
+			}
+			if (!rc && !graph && prof_end(stop))
+				rc = -1;
+		}
+		if (graph) {
+			const hipError_t e = hipStreamEndCapture(g_stream, &gr);
+			if (!rc && e != hipSuccess)
+				rc = fail("rotate_batched: end capture: %s",
+					hipGetErrorString(e));
+			if (!rc && hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0) !=
+				hipSuccess)
+				rc = fail("rotate_batched: graph instantiate");
+			if (!rc && hipGraphLaunch(ge, g_stream) != hipSuccess)
+				rc = fail("rotate_batched: graph launch");
+			if (!rc && hipStreamSynchronize(g_stream) != hipSuccess)
+				rc = fail("rotate_batched: graph sync");
+			if (ge)
+				(void)hipGraphExecDestroy(ge);
+			if (gr)
+				(void)hipGraphDestroy(gr);
+		}
+		if (!rc && launches)
+			*launches = maxp;
+	}
+out:
+	free(desc);
+	free(items);
+	free(first);
+	free(lstart);
+	free(ltiles);
+	return rc;
+}
+
+int ph2hip_scatter_set(void *amp, uint64_t namp_total, const uint64_t *idx,
+	const double *val, size_t n)
+{
+	if (ensure_init())
+		return -1;
+	if (n == 0)
+		return 0;
+	for (size_t k = 0; k < n; k++)
+		if (idx[k] >= namp_total)
+			return fail("scatter_set: index %llu beyond %llu",
+				(unsigned long long)idx[k],
+				(unsigned long long)namp_total);
+	// later entries win: keep the last occurrence of every index
+	size_t *ord = (size_t *)malloc(sizeof(size_t) * n);
+	uint64_t *ui = (uint64_t *)malloc(sizeof(uint64_t) * n);
+	double *uv = (double *)malloc(sizeof(double) * 2 * n);
+	if (!ord || !ui || !uv) {
+		free(ord);
+		free(ui);
+		free(uv);
+		return fail("scatter_set: out of host memory");
+	}
+	for (size_t k = 0; k < n; k++)
+		ord[k] = k;
+	std::stable_sort(ord, ord + n,
+		[&](size_t x, size_t y) { return idx[x] < idx[y]; });
+	size_t m = 0;
+	for (size_t k = 0; k < n; k++) {
+		if (k + 1 < n && idx[ord[k + 1]] == idx[ord[k]])
+			continue;
+		ui[m] = idx[ord[k]];
+		uv[2 * m] = val[2 * ord[k]];
+		uv[2 * m + 1] = val[2 * ord[k] + 1];
+		m++;
+	}
+	uint64_t *di = nullptr;
+	double2 *dv = nullptr;
+	int rc = 0;
+	if (hipMalloc(&di, sizeof(uint64_t) * m) != hipSuccess ||
+	    hipMalloc(&dv, sizeof(double2) * m) != hipSuccess)
+		rc = fail("scatter_set: hipMalloc");
+	if (!rc && (hipMemcpyAsync(di, ui, sizeof(uint64_t) * m,
+			    hipMemcpyHostToDevice, g_stream) != hipSuccess ||
+		    hipMemcpyAsync(dv, uv, sizeof(double2) * m,
+			    hipMemcpyHostToDevice, g_stream) != hipSuccess))
+		rc = fail("scatter_set: copy");
+	if (!rc) {
+		hipLaunchKernelGGL(k_scatter_set, dim3(grid_for(m, kBlock, kMaxGrid)),
+			dim3(kBlock), 0, g_stream, (double2 *)amp, di, dv,
+			(uint64_t)m);
+		if (hipGetLastError() != hipSuccess)
+			rc = fail("scatter_set: launch");
+	}
+	if (hipStreamSynchronize(g_stream) != hipSuccess && !rc)
+		rc = fail("scatter_set: sync");
+	(void)hipFree(di);
+	(void)hipFree(dv);
+	free(ord);
+	free(ui);
+	free(uv);
+	return rc;
+}
+
+int ph2hip_gather_abs(const void *amp, uint64_t namp_total,
+	const uint64_t *idx, size_t n, double *out)
+{
+	for (size_t k = 0; k < n; k++)
+		if (idx[k] >= namp_total)
+			return fail("gather: index beyond the buffer");
+	// ph2hip_gather checks against 2^nqb; pass the buffer's bit width
+	uint32_t bits = 0;
+	while ((UINT64_C(1) << bits) < namp_total)
+		bits++;
+	return ph2hip_gather(amp, bits, idx, n, out);
+}
+
 } // extern "C"
+

@@ -175,3 +175,77 @@ def trott_n33():
     hm = hamiltonian(33, 0, seed=0, decay=True, n_total=100_000)
     idx, cf = multidet_random(33, 64, 16, seed=0)
     return dict(hm=hm, det_idx=idx, det_cf=cf, delta=0.02, steps=8)
+
+
+def hamiltonian_vec(n: int, n_four: int, seed: int, *, sigma_z: float = 0.5,
+                    sigma_zz: float = 0.05, sigma_4: float = 0.01,
+                    decay: bool = True) -> Hamiltonian:
+    """Vectorised variant of ``hamiltonian`` for many registers: same term
+    families and distributions, but the four-index supports come from the
+    first four columns of a random permutation of each row (numpy argsort
+    of uniform keys), patterns and coefficients are drawn in bulk, and
+    duplicate strings are merged away before topping up.  Used for the
+    batched qDRIFT workload; its stream differs from ``hamiltonian``."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    p_, q_ = np.triu_indices(n, 1)
+    zs = [np.uint64(1) << np.arange(n, dtype=np.uint64),
+          (np.uint64(1) << p_.astype(np.uint64)) |
+          (np.uint64(1) << q_.astype(np.uint64))]
+    cs = [rng.normal(0.0, sigma_z, n), rng.normal(0.0, sigma_zz, len(p_))]
+    pat_y = np.array([[c == "Y" for c in p] for p in PATTERNS])
+    bits = np.uint64(1) << np.arange(n, dtype=np.uint64)
+    below = np.concatenate([[0], np.cumsum(bits)]).astype(np.uint64)
+    seen_x = np.zeros(0, np.uint64)
+    seen_z = np.zeros(0, np.uint64)
+    seen_c = np.zeros(0)
+    while len(seen_x) < n_four:
+        k = 2 * (n_four - len(seen_x)) + 16
+        sup = np.sort(np.argsort(rng.random((k, n)), axis=1)[:, :4], axis=1)
+        pat = rng.integers(len(PATTERNS), size=k)
+        c = rng.normal(0.0, sigma_4, k)
+        p, q, r, s = sup.T
+        if decay:
+            c = c * np.exp(-(r - p) / 8.0)
+        xm = bits[p] | bits[q] | bits[r] | bits[s]
+        y = pat_y[pat]
+        zm = (np.where(y[:, 0], bits[p], 0) | np.where(y[:, 1], bits[q], 0) |
+              np.where(y[:, 2], bits[r], 0) | np.where(y[:, 3], bits[s], 0))
+        # Z strictly between p..q and r..s: bits (p, q) = below[q] ^ below[p+1]
+        zm = zm | (below[q] ^ below[p + 1]) | (below[s] ^ below[r + 1])
+        zm = zm.astype(np.uint64)
+        allx = np.concatenate([seen_x, xm])
+        allz = np.concatenate([seen_z, zm])
+        allc = np.concatenate([seen_c, c])
+        _, first = np.unique(np.stack([allx, allz], 1), axis=0,
+                             return_index=True)
+        keep = np.sort(first)[:n_four]
+        seen_x, seen_z, seen_c = allx[keep], allz[keep], allc[keep]
+    xm = np.concatenate([np.zeros(n + len(p_), np.uint64), seen_x,
+                         np.zeros(1, np.uint64)])
+    zm = np.concatenate(zs + [seen_z, np.zeros(1, np.uint64)])
+    cf = np.concatenate(cs + [seen_c, rng.normal(0.0, sigma_z, 1)])
+    order = np.argsort(-np.abs(cf), kind="stable")
+    xm, zm, cf = xm[order], zm[order], cf[order]
+    return Hamiltonian(n, xm, zm, cf / np.sum(np.abs(cf)))
+
+
+def qdrift_batch(nreg: int = 512, seed: int = 0, samples: int = 16,
+                 depth: int = 1000, step_size: float = 0.01,
+                 nmin: int = 16, nmax: int = 26):
+    """Batched qDRIFT workload: ``nreg`` registers of widths drawn uniformly
+    from [nmin, nmax], each with its own Hamiltonian of 2,000-10,000 terms
+    (``hamiltonian_vec``), a 4-determinant initial state, and its own
+    xoshiro256** seed; ``samples`` samples of ``depth`` draws each."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    widths = rng.integers(nmin, nmax + 1, size=nreg)
+    totals = rng.integers(2000, 10001, size=nreg)
+    regs = []
+    for i in range(nreg):
+        n = int(widths[i])
+        nfixed = n + n * (n - 1) // 2 + 1
+        hm = hamiltonian_vec(n, int(totals[i]) - nfixed, seed=seed * 7919 + i)
+        idx, cf = multidet_random(n, 4, n // 2, seed=seed * 104729 + i)
+        regs.append(dict(hm=hm, det_idx=idx, det_cf=cf, samples=samples,
+                         depth=depth, step_size=step_size,
+                         seed=0x9E3779B97F4A7C15 ^ (seed << 20) ^ (i + 1)))
+    return regs

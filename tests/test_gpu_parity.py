@@ -297,3 +297,34 @@ def test_fused_pipelined_form_bit_identical(oracle, pipe_tile):
     finally:
         nat.set_fused_form(0, 0)
         nat.set_plain_form(2)
+
+
+def _batch_regs(widths, depth=120, samples=2):
+    regs = []
+    for i, n in enumerate(widths):
+        hm = synth.hamiltonian(n, 40 + 7 * n, seed=100 + i)
+        idx, cf = synth.multidet_random(n, 3, n // 2, seed=200 + i)
+        regs.append(dict(hm=hm, det_idx=idx, det_cf=cf, samples=samples,
+                         depth=depth, step_size=0.03, seed=0x9000 + i))
+    return regs
+
+
+@pytest.mark.parametrize("mode,tile", [(nat.BATCH_PLAIN_EACH, 12),
+                                       (nat.BATCH_FUSED_EACH, 12),
+                                       (nat.BATCH_FUSED_BATCHED, 11),
+                                       (nat.BATCH_FUSED_BATCHED, 0)])
+def test_batch_qdrift_vs_oracle(oracle, mode, tile):
+    """Every register of a batch of uneven widths gives exactly the
+    overlaps of qdrift run on it alone (same seed)."""
+    regs = _batch_regs([12, 9, 14, 11, 13, 12])
+    got, st = nat.batch_qdrift(regs, mode, tile, 4)
+    print("batch", mode, tile, st, flush=True)
+    for r, g in zip(regs, got):
+        want, _ = oracle.qdrift(r["hm"], dict(det_idx=r["det_idx"],
+                                              det_cf=r["det_cf"]),
+                                r["samples"], r["depth"], r["step_size"],
+                                r["seed"], want_state=False)
+        assert np.array_equal(g, want)
+    if mode == nat.BATCH_FUSED_BATCHED:
+        # one launch per pass index instead of one per pass per register
+        assert st["launches"] < st["passes"]
