@@ -130,6 +130,10 @@ def main(argv=None):
     ap.add_argument("--tile", type=int, default=12)
     ap.add_argument("--low", type=int, default=4)
     ap.add_argument("--plain-terms", type=int, default=24)
+    ap.add_argument("--schedule", choices=("auto", "greedy", "learned"),
+                    default="auto",
+                    help="fusion schedule; auto = learned when the trained "
+                    "parameters (phase2_amd/costmodel_params.npz) exist")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-baseline", dest="cpu", action="store_true",
                     default=True)
@@ -168,13 +172,29 @@ def main(argv=None):
 
     namp = 2.0 ** nqb
     nat.init()
+    from phase2_amd import costmodel
+    schedule = args.schedule
+    if schedule == "auto":
+        schedule = "learned" if costmodel.PARAMS_PATH.is_file() else "greedy"
+    if schedule == "learned":
+        params, _ = costmodel.load()
+        costmodel.install(params, low=args.low, min_qb=min(nqb, 28))
     log(f"bench: allocating {nqb}-qubit register ({16 * namp / 2**30:.0f} GiB)")
     reg = nat.Register(nqb, nat.FUSED, args.tile, args.low)
     reg.prep_multidet(w["det_idx"], w["det_cf"])
 
-    for k in range(args.warmup):
+    # Warm-up.  With the learned schedule every timed block is visited once
+    # first, so its schedule is computed and cached as in steps 2.. of a
+    # Trotter run; the one-off scheduling time is reported separately.
+    sched_s = 0.0
+    nwarm = max(args.warmup, len(blocks) if schedule == "learned" else 0)
+    for k in range(nwarm):
+        ts = time.perf_counter()
         reg.rotate(*block(blocks[k % len(blocks)]))
-        log(f"bench: warmup {k + 1}/{args.warmup}")
+        nat.synchronize()
+        if schedule == "learned" and k < len(blocks):
+            sched_s += time.perf_counter() - ts
+        log(f"bench: warmup {k + 1}/{nwarm}")
     nat.synchronize()
 
     nat.prof_enable(True)
@@ -227,6 +247,8 @@ def main(argv=None):
 
     norm2 = reg.norm2()
     reg.free()
+    if schedule == "learned":
+        costmodel.uninstall()
 
     cpu = None
     if args.cpu:
@@ -258,7 +280,7 @@ def main(argv=None):
             "blocks_timed": blocks,
             "blocks_in_sweep": nblocks,
             "delta": w["delta"],
-            "schedule": "greedy",
+            "schedule": schedule,
             "tile_bits": args.tile,
             "low_bits": args.low,
         },
@@ -282,6 +304,7 @@ def main(argv=None):
         "plain": plain,
         "fused_vs_plain": (value / plain["amp_updates_per_s"]) if plain else None,
         "norm_drift": abs(1.0 - norm2),
+        "first_visit_seconds": sched_s if schedule == "learned" else None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -90,6 +90,12 @@ int ph2hip_rotate_fused(void *amp, uint32_t nqb, const struct ph2hip_rot *rot,
  * workgroup per tile.  Initial mask from PHASE2_HIP_PIPE, else 0. */
 int ph2hip_set_fused_form(uint32_t pipe_mask, uint64_t persistent_wgs);
 
+/* Shape of the fused pass over rot[0..n) without launching it: shape[0] =
+ * contiguous low bits L, shape[1] = gathered dimensions g, shape[2] =
+ * micro-groups.  -1 if the terms do not fit one tile. */
+int ph2hip_pass_shape(const struct ph2hip_rot *rot, size_t n,
+	uint32_t tile_bits, uint32_t low_bits, uint32_t *shape);
+
 /* Dimension of span{x_k & ~(2^low_bits - 1)} over GF(2) for k in [0, n):
  * the number of tile bits beyond low_bits a pass over these terms needs. */
 uint32_t ph2hip_span_rank(const struct ph2hip_rot *rot, size_t n,
@@ -196,6 +202,28 @@ int ph2hip_elapsed_ms(int slot_start, int slot_stop, double *ms);
 
 /* sum_i |amp[i]|^2 (norm-drift reporting), fixed reduction order. */
 int ph2hip_norm2(const void *amp, uint32_t nqb, double *out);
+
+/* -- cost-model network ----------------------------------------------------- */
+
+/* FP32 perceptron nin -> nhid -> nhid -> 1 with ReLU (mlp.hip): forward,
+ * backward and Adam run as HIP kernels on their own stream.  Parameters are
+ * packed W1[nhid][nin], b1, W2[nhid][nhid], b2, W3[nhid], b3; the loss is
+ * 0.5 * mean (y - t)^2.  Batches are host arrays of at most max_batch rows
+ * (forward splits larger inputs).  No counterpart in the reference. */
+struct ph2hip_mlp;
+int ph2hip_mlp_create(struct ph2hip_mlp **m, uint32_t nin, uint32_t nhid,
+	uint32_t max_batch, uint64_t seed);
+void ph2hip_mlp_destroy(struct ph2hip_mlp *m);
+size_t ph2hip_mlp_nparams(const struct ph2hip_mlp *m);
+int ph2hip_mlp_set_params(struct ph2hip_mlp *m, const float *params);
+int ph2hip_mlp_get_params(struct ph2hip_mlp *m, float *params);
+int ph2hip_mlp_forward(struct ph2hip_mlp *m, const float *x, size_t batch,
+	float *y);
+int ph2hip_mlp_grad(struct ph2hip_mlp *m, const float *x, const float *t,
+	size_t batch, float *loss, float *grad);
+int ph2hip_mlp_train_step(struct ph2hip_mlp *m, const float *x,
+	const float *t, size_t batch, float lr, float *loss);
+const char *ph2hip_mlp_strerror(void);
 
 #ifdef __cplusplus
 }

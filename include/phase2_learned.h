@@ -1,0 +1,64 @@
+#ifndef PHASE2_LEARNED_H
+#define PHASE2_LEARNED_H
+
+/*
+ * Learned fusion schedule (no counterpart in the reference; the greedy cut
+ * of ph2_sched_greedy is the fallback).  An FP32 perceptron (ph2hip_mlp_*)
+ * predicts the time of a candidate fused pass from
+ *   f[0..40)   bit k of the OR of the pass's flip masks
+ *   f[40..80)  bit k of the OR of its phase masks
+ *   f[80]      terms / 48        f[81] micro-groups / 48
+ *   f[82]      gathered dims / 13  f[83] contiguous low bits / 13
+ *   f[84]      tile bits / 13
+ * as y = ln(ms) - (nqb - 30) ln 2 (time per 2^30 amplitudes).  For a block of
+ * terms the scheduler enumerates passes [i, j) that end where the next term
+ * would widen the tile (or at the longest feasible end), for tile widths 11
+ * and 12, predicts them all in one batch on the GPU and picks the cut with
+ * the smallest predicted total by dynamic programming.  Schedules are cached
+ * per block (flip and phase masks, width), so repeated Trotter sweeps pay
+ * for scheduling once.  The order of terms never changes, so the state stays
+ * bit-identical to the plain sweep.
+ */
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "phase2/qreg_hip.h"
+#include "phase2_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PH2_LM_NQ 40
+#define PH2_LM_NIN (2 * PH2_LM_NQ + 5)
+
+/* Feature vector (PH2_LM_NIN floats) of the pass rot[0..n) in tiles of
+ * tile_bits with the given shape (ph2hip_pass_shape). */
+void ph2_learned_features(const struct ph2hip_rot *rot, size_t n,
+	uint32_t tile_bits, const uint32_t *shape, float *f);
+
+/* Load parameters (packed as ph2hip_mlp) of a PH2_LM_NIN -> nhid -> nhid
+ * -> 1 network, and install the learned schedule for fused registers of at
+ * least min_qb qubits (narrower ones keep the greedy cut).  low_bits is
+ * the requested contiguous low part of every tile. */
+int ph2_learned_install(const float *params, uint32_t nhid, uint32_t low_bits,
+	uint32_t min_qb);
+void ph2_learned_uninstall(void);
+
+/* Learned schedule of one block (needs ph2_learned_install). */
+int ph2_learned_schedule(const struct ph2hip_rot *rot, size_t n, uint32_t nqb,
+	struct ph2_sched *out);
+
+/* Predicted milliseconds of each pass of a schedule (ms[npass]). */
+int ph2_learned_predict(const struct ph2hip_rot *rot, size_t n, uint32_t nqb,
+	const struct ph2_sched *s, double *ms);
+
+/* Cache statistics: hits and misses since install. */
+void ph2_learned_stats(uint64_t *hits, uint64_t *misses);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHASE2_LEARNED_H */

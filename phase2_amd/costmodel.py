@@ -1,0 +1,291 @@
+"""Learned cost model of fused passes (include/phase2_learned.h).
+
+An FP32 perceptron (85 -> 64 -> 64 -> 1, ReLU) predicts the time of a fused
+pass from the mask-bit features of its terms; the scheduler in
+csrc/sched_learned.c uses it to choose pass boundaries and tile widths.  The
+network is trained on the GPU by the library's own kernels (mlp.hip:
+forward, backward, Adam) from per-pass timings the library records while
+profiling (ph2hip_prof_records).
+
+Feature vector (must match ph2_learned_features): bits 0..39 of the OR of
+the pass's flip masks, bits 0..39 of the OR of its phase masks, then
+terms/48, micro-groups/48, gathered dims/13, contiguous low bits/13,
+tile bits/13.  Target: ln(ms) - (nqb - 30) ln 2.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from pathlib import Path
+
+import numpy as np
+
+from . import _native as nat
+
+NQ = 40
+NIN = 2 * NQ + 5
+NHID = 64
+PARAMS_PATH = Path(__file__).resolve().parent / "costmodel_params.npz"
+LN2 = math.log(2.0)
+
+_f32p = C.POINTER(C.c_float)
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+class MLP:
+    """The library's FP32 network on the GPU (ph2hip_mlp_*)."""
+
+    def __init__(self, nin=NIN, nhid=NHID, max_batch=8192, seed=1):
+        L = nat.lib()
+        L.ph2hip_mlp_create.restype = C.c_int
+        L.ph2hip_mlp_nparams.restype = C.c_size_t
+        L.ph2hip_mlp_strerror.restype = C.c_char_p
+        for f in ("ph2hip_mlp_set_params", "ph2hip_mlp_get_params",
+                  "ph2hip_mlp_forward", "ph2hip_mlp_grad",
+                  "ph2hip_mlp_train_step"):
+            getattr(L, f).restype = C.c_int
+        self.L = L
+        self.nin, self.nhid, self.max_batch = nin, nhid, max_batch
+        h = C.c_void_p()
+        if L.ph2hip_mlp_create(C.byref(h), C.c_uint32(nin), C.c_uint32(nhid),
+                               C.c_uint32(max_batch), C.c_uint64(seed)) != 0:
+            self._err("ph2hip_mlp_create")
+        self.h = h
+        self.np = int(L.ph2hip_mlp_nparams(h))
+
+    def _err(self, what):
+        msg = self.L.ph2hip_mlp_strerror().decode(errors="replace")
+        raise RuntimeError(f"{what} failed: {msg} / {nat.lib().ph2_strerror()}")
+
+    def get_params(self):
+        p = np.zeros(self.np, np.float32)
+        if self.L.ph2hip_mlp_get_params(self.h, _p(p, _f32p)) != 0:
+            self._err("get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.shape == (self.np,)
+        if self.L.ph2hip_mlp_set_params(self.h, _p(p, _f32p)) != 0:
+            self._err("set_params")
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.zeros(X.shape[0], np.float32)
+        if self.L.ph2hip_mlp_forward(self.h, _p(X, _f32p),
+                                     C.c_size_t(X.shape[0]), _p(y, _f32p)) != 0:
+            self._err("forward")
+        return y
+
+    def grad(self, X, t):
+        X = np.ascontiguousarray(X, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(self.np, np.float32)
+        loss = C.c_float()
+        if self.L.ph2hip_mlp_grad(self.h, _p(X, _f32p), _p(t, _f32p),
+                                  C.c_size_t(X.shape[0]), C.byref(loss),
+                                  _p(g, _f32p)) != 0:
+            self._err("grad")
+        return float(loss.value), g
+
+    def train_step(self, X, t, lr):
+        X = np.ascontiguousarray(X, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        loss = C.c_float()
+        if self.L.ph2hip_mlp_train_step(self.h, _p(X, _f32p), _p(t, _f32p),
+                                        C.c_size_t(X.shape[0]),
+                                        C.c_float(lr), C.byref(loss)) != 0:
+            self._err("train_step")
+        return float(loss.value)
+
+    def close(self):
+        if self.h:
+            self.L.ph2hip_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def unpack(p, nin=NIN, nhid=NHID):
+    """Split packed parameters into (W1, b1, W2, b2, W3, b3)."""
+    o = 0
+    out = []
+    for shape in ((nhid, nin), (nhid,), (nhid, nhid), (nhid,), (nhid,), (1,)):
+        n = int(np.prod(shape))
+        out.append(np.asarray(p[o:o + n]).reshape(shape))
+        o += n
+    return out
+
+
+# -- features ------------------------------------------------------------------
+
+def _bits(v, n=NQ):
+    v = np.asarray(v, dtype=np.uint64)
+    return ((v[:, None] >> np.arange(n, dtype=np.uint64)) & np.uint64(1)
+            ).astype(np.float32)
+
+
+def features_from_records(rec):
+    """Feature matrix and targets of fused-pass records (REC_DTYPE)."""
+    rec = rec[rec["kind"] == 2]
+    X = np.zeros((len(rec), NIN), np.float32)
+    X[:, :NQ] = _bits(rec["xunion"])
+    X[:, NQ:2 * NQ] = _bits(rec["zunion"])
+    X[:, 2 * NQ + 0] = rec["nterms"] / 48.0
+    X[:, 2 * NQ + 1] = rec["ngroups"] / 48.0
+    X[:, 2 * NQ + 2] = rec["gdim"] / 13.0
+    X[:, 2 * NQ + 3] = rec["low_bits"] / 13.0
+    X[:, 2 * NQ + 4] = rec["tile_bits"] / 13.0
+    y = (np.log(rec["ms"]) - (rec["nqb"].astype(np.float64) - 30) * LN2
+         ).astype(np.float32)
+    return X, y, rec
+
+
+def c_features(xm, zm, tile_bits, shape):
+    """ph2_learned_features on one pass (for checking the Python mirror)."""
+    n = len(xm)
+    rot = (nat.Rot * max(n, 1))()
+    for k in range(n):
+        rot[k] = nat.Rot(int(xm[k]), int(zm[k]), 1.0, 0.0)
+    sh = (C.c_uint32 * 3)(*shape)
+    f = np.zeros(NIN, np.float32)
+    nat.lib().ph2_learned_features(rot, C.c_size_t(n), C.c_uint32(tile_bits),
+                                   sh, _p(f, _f32p))
+    return f
+
+
+# -- data collection -------------------------------------------------------------
+
+def _span_growth(xs, low):
+    """rank[k] = dim span{x_0..x_k} mod the low bits."""
+    basis = {}
+    mask = ~((1 << low) - 1)
+    out = []
+    for x in xs:
+        v = int(x) & mask
+        while v:
+            h = v.bit_length() - 1
+            if h in basis:
+                v ^= basis[h]
+            else:
+                basis[h] = v
+                break
+        out.append(len(basis))
+    return out
+
+
+def random_schedule(xm, rng, tiles=(11, 12), low=4, maxlen=48):
+    """Cut a block into passes of random feasible length and tile width."""
+    lens, tbs = [], []
+    i, n = 0, len(xm)
+    while i < n:
+        tb = int(rng.choice(tiles))
+        ranks = _span_growth(xm[i:i + maxlen], low)
+        jmax = 1
+        while jmax < len(ranks) and ranks[jmax] <= tb - low:
+            jmax += 1
+        ln = int(rng.integers(1, jmax + 1)) if rng.random() < 0.7 else jmax
+        lens.append(ln)
+        tbs.append(tb)
+        i += ln
+    return np.array(lens, np.uint32), np.array(tbs, np.uint32)
+
+
+def collect(reg, xm, zm, phi, blocks, rng, low=4, log=print):
+    """Run random schedules over the given 1024-term blocks on a device
+    register and return the per-pass timing records."""
+    L = nat.lib()
+    L.ph2_reg_rotate_sched.restype = C.c_int
+    nat.prof_enable(True)
+    nat.prof_records(reset=True)
+    t0 = time.perf_counter()
+    for bi, b in enumerate(blocks):
+        s = slice(b * 1024, (b + 1) * 1024)
+        bx, bz, bp = (np.ascontiguousarray(xm[s]), np.ascontiguousarray(zm[s]),
+                      np.ascontiguousarray(phi[s]))
+        lens, tbs = random_schedule(bx, rng, low=low)
+        rc = L.ph2_reg_rotate_sched(
+            C.c_void_p(reg.h), C.c_size_t(len(bx)),
+            bx.ctypes.data_as(C.POINTER(C.c_uint64)),
+            bz.ctypes.data_as(C.POINTER(C.c_uint64)),
+            bp.ctypes.data_as(C.POINTER(C.c_double)),
+            lens.ctypes.data_as(C.POINTER(C.c_uint32)),
+            tbs.ctypes.data_as(C.POINTER(C.c_uint32)), C.c_size_t(len(lens)),
+            C.c_uint32(low))
+        if rc != 0:
+            raise RuntimeError(f"ph2_reg_rotate_sched: {L.ph2_strerror()}")
+        if (bi + 1) % 8 == 0:
+            nat.synchronize()
+            log(f"collect: block {bi + 1}/{len(blocks)} "
+                f"{time.perf_counter() - t0:.1f}s")
+    rec = nat.prof_records(reset=True)
+    nat.prof_enable(False)
+    return rec
+
+
+# -- training ------------------------------------------------------------------
+
+def train(X, y, epochs=200, batch=512, lr=2e-3, seed=0, val_frac=0.1,
+          log=print):
+    """Train the network on the GPU; returns (params, history)."""
+    rng = np.random.default_rng(seed)
+    n = len(y)
+    perm = rng.permutation(n)
+    nval = max(1, int(n * val_frac))
+    va, tr = perm[:nval], perm[nval:]
+    m = MLP(seed=seed + 1)
+    hist = []
+    for ep in range(epochs):
+        order = rng.permutation(tr)
+        lr_ep = lr * (0.5 * (1 + math.cos(math.pi * ep / epochs)))
+        tot = 0.0
+        for k in range(0, len(order), batch):
+            idx = order[k:k + batch]
+            tot += m.train_step(X[idx], y[idx], lr_ep) * len(idx)
+        if ep % 20 == 0 or ep == epochs - 1:
+            pv = m.forward(X[va])
+            vl = float(np.mean(0.5 * (pv - y[va]) ** 2))
+            hist.append((ep, tot / len(tr), vl))
+            log(f"train: epoch {ep} loss {tot / len(tr):.5f} val {vl:.5f}")
+    p = m.get_params()
+    m.close()
+    return p, dict(history=hist, val_idx=va)
+
+
+def save(params, path=PARAMS_PATH, **meta):
+    np.savez(path, params=np.asarray(params, np.float32), nin=NIN, nhid=NHID,
+             **meta)
+
+
+def load(path=PARAMS_PATH):
+    d = np.load(path, allow_pickle=False)
+    return d["params"], {k: d[k] for k in d.files if k != "params"}
+
+
+def install(params, low=4, min_qb=28):
+    """Use the learned schedule for fused registers of >= min_qb qubits."""
+    p = np.ascontiguousarray(params, np.float32)
+    L = nat.lib()
+    L.ph2_learned_install.restype = C.c_int
+    if L.ph2_learned_install(_p(p, _f32p), C.c_uint32(NHID), C.c_uint32(low),
+                             C.c_uint32(min_qb)) != 0:
+        raise RuntimeError(f"ph2_learned_install: {L.ph2_strerror()}")
+
+
+def uninstall():
+    nat.lib().ph2_learned_uninstall()
+
+
+def stats():
+    h, m = C.c_uint64(), C.c_uint64()
+    nat.lib().ph2_learned_stats(C.byref(h), C.byref(m))
+    return int(h.value), int(m.value)

@@ -1,6 +1,7 @@
 /*
  * Flat-argument entry points (include/phase2_flat.h) over the phase2 API.
  */
+#define _GNU_SOURCE /* sincos */
 #define LOG_SUBSYS "flat"
 
 #include <complex.h>
@@ -352,6 +353,25 @@ int ph2_reg_rotate(void *r, size_t n, const uint64_t *xm, const uint64_t *zm,
 			return -1;
 	}
 	return 0;
+}
+
+int ph2_reg_rotate_sched(void *r, size_t n, const uint64_t *xm,
+	const uint64_t *zm, const double *phi, const uint32_t *pass_len,
+	const uint32_t *tile_bits, size_t npass, uint32_t low_bits)
+{
+	struct qreg *reg = r;
+	struct ph2hip_rot *rot = malloc(sizeof *rot * (n ? n : 1));
+	if (!rot)
+		return -1;
+	for (size_t k = 0; k < n; k++) {
+		rot[k].x = xm[k];
+		rot[k].z = zm[k];
+		sincos(phi[k], &rot[k].s, &rot[k].c);
+	}
+	const int rc = ph2hip_rotate_fused(qreg_hip_device_ptr(reg), reg->qb_lo,
+		rot, n, pass_len, tile_bits, npass, low_bits);
+	free(rot);
+	return rc;
 }
 
 /* sum_d amp[idx_d] * conj(cf_d) in determinant order, as circ_measure. */

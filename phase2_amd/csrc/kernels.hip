@@ -1795,5 +1795,20 @@ int ph2hip_gather_abs(const void *amp, uint64_t namp_total,
 	return ph2hip_gather(amp, bits, idx, n, out);
 }
 
+int ph2hip_pass_shape(const struct ph2hip_rot *rot, size_t n,
+	uint32_t tile_bits, uint32_t low_bits, uint32_t *shape)
+{
+	if (n == 0 || n > PH2HIP_PASS_MAX || tile_bits < PH2HIP_TILE_MIN ||
+	    tile_bits > PH2HIP_TILE_MAX)
+		return -1;
+	FusedPass P;
+	if (build_pass(P, rot, n, tile_bits, low_bits, 0))
+		return -1;
+	shape[0] = P.L;
+	shape[1] = P.g;
+	shape[2] = P.ngroups;
+	return 0;
+}
+
 } // extern "C"
 
