@@ -1,25 +1,41 @@
 #!/usr/bin/env python3
-"""Benchmark: Pauli amp-updates/s and % of HBM peak of the 33-qubit FP64
-1st-order Trotter sweep on one MI355X, fused (LDS coset tiles) against plain
-(one HBM pass per term), with the reference CPU backend timed beside it.
+"""Benchmark: Pauli amp-updates/s and % of HBM peak of phase2's state-vector
+sweep on one MI355X, fused/batched against plain, with the reference CPU
+backend timed beside it.
 
-Workload (synthetic, phase2_amd.synth.trott_n33): 33 qubits (128 GiB
-complex128 register resident in HBM), 100,000 terms (33 Z, 528 ZZ, 99,438
-four-index, identity), lexicographic order as trott_init sorts it,
-delta = 0.02, 64-determinant initial state.  The sweep is applied in
-circ_cache blocks of 1024 consecutive terms through qreg_paulirot.  A step is
-one whole block; the timed steps cycle through 8 blocks spread evenly over
-the sweep (a full sweep is 98 blocks), because the blocks differ in how
-well they fuse (the first block, with all flip masks on qubits 0..11, fuses
-into one pass).
+Workloads (synthetic, phase2_amd.synth; --workload):
 
-value        whole-job amp-updates/s of the fused sweep over the timed steps
-roofline     the fused kernel (k_rot_fused): algorithmic bytes per launch =
-             32 B x 2^n (one read and one write of the register) divided by
-             its mean launch duration (HIP events on the library stream)
-plain        the plain kernel on terms spread evenly through the same blocks
-cpu_baseline the reference CPU backend (oracle/_ref/ref_bench, built from the
-             reference sources) under mpirun on this host
+trott_n33 (default, the headline)
+    33 qubits (128 GiB complex128 register resident in HBM), 100,000 terms
+    (33 Z, 528 ZZ, 99,438 four-index, identity) in the lexicographic order
+    trott_init sorts them into, delta = 0.02, 64-determinant initial state.
+    The sweep is applied in circ_cache blocks of 1024 consecutive terms
+    through qreg_paulirot.  A step is one whole block; the timed steps cycle
+    through 8 blocks spread evenly over the sweep (a full sweep is 98 blocks),
+    because the blocks differ in how well they fuse (the first block, with
+    all flip masks on qubits 0..11, fuses into one pass).
+trott_n16
+    1st-order Trotter at 16 qubits, 537 terms, delta 0.1; a step is one
+    Trotter step (sweep + overlap), the whole trott_simul is timed.
+trott2_n26
+    2nd-order Trotter at 26 qubits, 20,352 terms, coefficient-matrix state
+    (230,230 determinants); a step is one trott2 step (forward and reverse
+    half-sweeps + coefficient-matrix overlap).
+qdrift_batch
+    512 qDRIFT registers of 16..26 qubits (107 GiB in one device buffer),
+    depth 1000; a step is one sample of every register, run as batched
+    launches (one per pass index) captured in a hipGraph per round.
+cmpsit_n30
+    composite integrator at 30 qubits, 60,000 terms, 2,000 deterministic +
+    512 sampled rotations per half-step; a step is one cmpsit step.
+
+value        whole-job amp-updates/s over the timed steps (inputs resident)
+roofline     the dominant kernel: algorithmic bytes per launch = 32 B per
+             amplitude swept (one read and one write of the register) divided
+             by its mean launch duration (HIP events on the library stream)
+plain        the plain one-term-per-pass kernel on terms of the same workload
+cpu_baseline the reference CPU backend (oracle/_ref, built from the reference
+             sources) under mpirun on this host, on a bounded sample
 """
 
 from __future__ import annotations
@@ -29,6 +45,7 @@ import json
 import math
 import os
 import shutil
+import struct
 import subprocess
 import sys
 import tempfile
@@ -44,14 +61,178 @@ METRIC = ("Pauli amp-updates/s + % HBM peak, 33-qb FP64 Trotter: "
           "fused/batched vs plain")
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_AMP_PASS = 32
+WORKLOADS = ("trott_n33", "trott_n16", "trott2_n26", "qdrift_batch",
+             "cmpsit_n30")
+DEFAULT_STEPS = dict(trott_n33=8, trott_n16=64, trott2_n26=4,
+                     qdrift_batch=16, cmpsit_n30=2)
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def workload(nqb: int):
+# -- reference CPU baselines ---------------------------------------------------
+
+def _mpi_runs(exe, args, ranks, timeout=240):
+    """Run a reference program under mpirun with `ranks` ranks (then with
+    one process if that fails); returns the last JSON line it printed."""
+    mpirun = shutil.which("mpirun") or "/opt/conda/bin/mpirun"
+    cmds = []
+    if ranks > 1 and Path(mpirun).exists():
+        cmds.append([mpirun, "-launcher", "fork", "-n", str(ranks), str(exe)]
+                    + args)
+    cmds.append([str(exe)] + args)
+    for cmd in cmds:
+        try:
+            r = subprocess.run(cmd, capture_output=True, text=True,
+                               timeout=timeout)
+        except subprocess.TimeoutExpired:
+            log("cpu baseline: timeout", cmd[:6])
+            continue
+        lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+        if r.returncode == 0 and lines:
+            return json.loads(lines[-1])
+        log("cpu baseline: failed", r.returncode, r.stderr[-400:])
+    return None
+
+
+def cpu_rotations(nqb, xm, zm, phi, ranks, what):
+    """Reference qreg_paulirot (oracle/_ref/ref_bench) on the rotations
+    (xm, zm, phi) applied in order to a `nqb`-qubit register; falls back to
+    the oracle port on one core when the reference build is absent."""
+    n = len(phi)
+    rec = np.zeros(n, dtype=[("x", "<u8"), ("z", "<u8"), ("phi", "<f8")])
+    rec["x"], rec["z"], rec["phi"] = xm, zm, phi
+    sample = f"{n} rotations ({what}) on a {nqb}-qubit register"
+    exe = ROOT / "oracle" / "_ref" / "ref_bench"
+    with tempfile.TemporaryDirectory() as td:
+        f = Path(td) / "terms.bin"
+        rec.tofile(f)
+        d = _mpi_runs(exe, [str(f), str(nqb)], ranks) if exe.is_file() \
+            else None
+    if d is not None:
+        return dict(value=d["amp_updates_per_s"], unit="amp-updates/s",
+                    cores=d["ranks"], kind="reference",
+                    sample=sample + f", {d['ranks']} MPI ranks x 1 thread, "
+                    "reference qreg_paulirot incl. its per-flush "
+                    "exchange/mix/add", seconds=d["seconds"])
+    from oracle import pyoracle
+    o = pyoracle.oracle()
+    st = np.zeros(1 << nqb, dtype=np.complex128)
+    st[0] = 1
+    t0 = time.perf_counter()
+    o.rotate(st, rec["x"], rec["z"], rec["phi"])
+    dt = time.perf_counter() - t0
+    return dict(value=n * 2.0 ** nqb / dt, unit="amp-updates/s", cores=1,
+                kind="port", sample=sample + ", oracle port, 1 thread",
+                seconds=dt)
+
+
+def cpu_circ(algo, hm, det_idx, det_cf, ranks, what, steps=0, delta=0.0,
+             samples=0, depth=0, seed=0):
+    """Whole reference circuit run (oracle/_ref/ref_circ: trott_simul or
+    qdrift_simul) under mpirun; None when the reference build is absent."""
+    exe = ROOT / "oracle" / "_ref" / "ref_circ"
+    if not exe.is_file():
+        return None
+    hdr = struct.pack("<IIQQQQQQd", algo, hm.nqb, len(hm), len(det_idx),
+                      steps, samples, depth, seed, delta)
+    t = np.zeros(len(hm), dtype=[("x", "<u8"), ("z", "<u8"), ("c", "<f8")])
+    t["x"], t["z"], t["c"] = hm.xm, hm.zm, hm.cf
+    d = np.zeros(len(det_idx), dtype=[("i", "<u8"), ("re", "<f8"),
+                                      ("im", "<f8")])
+    d["i"] = det_idx
+    d["re"], d["im"] = np.real(det_cf), np.imag(det_cf)
+    with tempfile.TemporaryDirectory() as td:
+        f = Path(td) / "job.bin"
+        with open(f, "wb") as fh:
+            fh.write(hdr)
+            fh.write(t.tobytes())
+            fh.write(d.tobytes())
+        r = _mpi_runs(exe, [str(f)], ranks)
+    if r is None:
+        return None
+    return dict(value=r["amp_updates_per_s"], unit="amp-updates/s",
+                cores=r["ranks"], kind="reference",
+                sample=f"{what}, {r['ranks']} MPI ranks x 1 thread, whole "
+                "reference circuit run", seconds=r["seconds"])
+
+
+def cpu_ranks(args):
+    if args.cpu_ranks > 0:
+        return args.cpu_ranks
+    ncpu = min(os.cpu_count() or 1, 16)
+    return 1 << int(math.log2(ncpu))
+
+
+# -- helpers -------------------------------------------------------------------
+
+def lex(hm, nat):
+    perm = nat.sort_lex(hm.xm, hm.zm)
+    return hm.xm[perm], hm.zm[perm], hm.cf[perm]
+
+
+def roofline(kernel, prof):
+    """Roofline object of one profiled kernel kind."""
+    launches = max(1, prof["launches"])
+    ms = prof["ms"] / launches
+    nbytes = BYTES_PER_AMP_PASS * prof["amps"] / launches
+    gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return {"bound": "hbm", "kernel": kernel, "achieved": gbs,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+            "traffic": None, "algorithmic_bytes_per_launch": nbytes,
+            "mean_launch_ms": ms, "launches": prof["launches"]}
+
+
+def traffic_for(kernel, nqb):
+    """HBM bytes per launch measured by rocprofv3 (FETCH_SIZE, WRITE_SIZE,
+    calibrated) for this kernel and width, from profiles/traffic.json."""
+    f = ROOT / "profiles" / "traffic.json"
+    if not f.is_file():
+        return None
+    try:
+        d = json.loads(f.read_text())
+    except ValueError:
+        return None
+    e = d.get(f"{kernel}@n{nqb}")
+    return None if e is None else e.get("bytes_per_launch")
+
+
+def plain_rate(nat, nqb, xm, zm, phi):
+    """The plain kernel on the given rotations on a fresh register."""
+    reg = nat.Register(nqb, nat.PLAIN)
+    reg.prep_multidet(np.array([1], np.uint64), np.array([1.0 + 0j]))
+    reg.rotate(xm[:2], zm[:2], phi[:2])  # warm-up
+    nat.synchronize()
+    nat.prof_enable(True)
+    nat.prof_read(reset=True)
+    t0 = time.perf_counter()
+    reg.rotate(xm, zm, phi)
+    nat.synchronize()
+    t1 = time.perf_counter()
+    pp = nat.prof_read(reset=True)
+    nat.prof_enable(False)
+    reg.free()
+    launches = pp["plain_pair"]["launches"] + pp["plain_diag"]["launches"]
+    pms = (pp["plain_pair"]["ms"] + pp["plain_diag"]["ms"]) / max(1, launches)
+    gbs = BYTES_PER_AMP_PASS * 2.0 ** nqb / (pms * 1e-3) / 1e9
+    return dict(amp_updates_per_s=len(phi) * 2.0 ** nqb / (t1 - t0),
+                kernel_ms_per_pass=pms, kernel_GBps=gbs,
+                hbm_frac=gbs / HBM_PEAK_GBS, terms=len(phi),
+                pair_launches=pp["plain_pair"]["launches"],
+                diag_launches=pp["plain_diag"]["launches"])
+
+
+def spread(n, k):
+    return np.unique(np.linspace(0, n - 1, min(n, k)).astype(int))
+
+
+# -- workloads -----------------------------------------------------------------
+
+def bench_trott_n33(args, nat, costmodel, schedule):
     from phase2_amd import synth
+    nqb = args.nqb
+    log(f"bench: generating workload nqb={nqb}")
     if nqb == 33:
         w = synth.trott_n33()
     else:
@@ -61,105 +242,8 @@ def workload(nqb: int):
                                            + 1 + 20 * nqb * nqb))
         idx, cf = synth.multidet_random(nqb, 64, nqb // 2, seed=0)
         w = dict(hm=hm, det_idx=idx, det_cf=cf, delta=0.02, steps=8)
-    return w
-
-
-def cpu_baseline(nqb: int, nterms: int, ranks: int):
-    """Reference CPU backend under mpirun on a bounded sample: the first
-    `nterms` terms (lex order) of the same generator at `nqb` qubits."""
-    from phase2_amd import synth, _native
-    exe = ROOT / "oracle" / "_ref" / "ref_bench"
-    hm = synth.hamiltonian(nqb, 0, seed=0, decay=True,
-                           n_total=nqb + nqb * (nqb - 1) // 2 + 1 + 4000)
-    perm = _native.sort_lex(hm.xm, hm.zm)
-    # skip the first block's low-qubit terms: take terms from the middle
-    sel = perm[len(perm) // 2: len(perm) // 2 + nterms]
-    rec = np.zeros(nterms, dtype=[("x", "<u8"), ("z", "<u8"), ("phi", "<f8")])
-    rec["x"], rec["z"] = hm.xm[sel], hm.zm[sel]
-    rec["phi"] = 0.02 * hm.cf[sel]
-    sample = (f"{nterms} rotations (lex-ordered terms of the same generator) "
-              f"on a {nqb}-qubit register")
-    with tempfile.TemporaryDirectory() as td:
-        f = Path(td) / "terms.bin"
-        rec.tofile(f)
-        if exe.is_file():
-            mpirun = shutil.which("mpirun") or "/opt/conda/bin/mpirun"
-            cmds = []
-            if ranks > 1 and Path(mpirun).exists():
-                cmds.append(([mpirun, "-launcher", "fork", "-n", str(ranks),
-                              str(exe), str(f), str(nqb)], ranks))
-            cmds.append(([str(exe), str(f), str(nqb)], 1))
-            for cmd, nr in cmds:
-                try:
-                    r = subprocess.run(cmd, capture_output=True, text=True,
-                                       timeout=240)
-                except subprocess.TimeoutExpired:
-                    log("cpu baseline: timeout", cmd[:4])
-                    continue
-                lines = [ln for ln in r.stdout.splitlines()
-                         if ln.startswith("{")]
-                if r.returncode == 0 and lines:
-                    d = json.loads(lines[-1])
-                    return dict(value=d["amp_updates_per_s"],
-                                unit="amp-updates/s", cores=d["ranks"],
-                                kind="reference",
-                                sample=sample + f", {d['ranks']} MPI ranks x "
-                                "1 thread, reference qreg_paulirot incl. its "
-                                "per-flush exchange/mix/add",
-                                seconds=d["seconds"])
-                log("cpu baseline: failed", r.returncode, r.stderr[-400:])
-        # the oracle port, one core
-        from oracle import pyoracle
-        o = pyoracle.oracle()
-        st = np.zeros(1 << nqb, dtype=np.complex128)
-        st[0] = 1
-        t0 = time.perf_counter()
-        o.rotate(st, rec["x"], rec["z"], rec["phi"])
-        dt = time.perf_counter() - t0
-        return dict(value=nterms * 2.0 ** nqb / dt, unit="amp-updates/s",
-                    cores=1, kind="port", sample=sample + ", 1 thread",
-                    seconds=dt)
-
-
-def main(argv=None):
-    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--nqb", type=int, default=33)
-    ap.add_argument("--tile", type=int, default=12)
-    ap.add_argument("--low", type=int, default=4)
-    ap.add_argument("--plain-terms", type=int, default=24)
-    ap.add_argument("--schedule", choices=("auto", "greedy", "learned"),
-                    default="auto",
-                    help="fusion schedule; auto = learned when the trained "
-                    "parameters (phase2_amd/costmodel_params.npz) exist")
-    ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--cpu-baseline", dest="cpu", action="store_true",
-                    default=True)
-    ap.add_argument("--no-cpu-baseline", dest="cpu", action="store_false")
-    ap.add_argument("--cpu-nqb", type=int, default=28)
-    ap.add_argument("--cpu-terms", type=int, default=16)
-    ap.add_argument("--cpu-ranks", type=int, default=0)
-    args = ap.parse_args(argv)
-
-    world = int(os.environ.get("WORLD_SIZE", "1"))
-    if args.gpus != 1 or world != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, "
-              f"WORLD_SIZE={world}): the register lives on a single MI355X "
-              "and there is no multi-GPU path in this build.",
-              file=sys.stderr, flush=True)
-        sys.exit(2)
-
-    from phase2_amd import _native as nat
-
-    nqb = args.nqb
-    log(f"bench: generating workload nqb={nqb}")
-    w = workload(nqb)
-    hm = w["hm"]
-    perm = nat.sort_lex(hm.xm, hm.zm)
-    xm, zm = hm.xm[perm], hm.zm[perm]
-    phi = w["delta"] * hm.cf[perm]
+    xm, zm, cf = lex(w["hm"], nat)
+    phi = w["delta"] * cf
     nterms = len(phi)
     nblocks = (nterms + 1023) // 1024
     S = 8
@@ -171,14 +255,6 @@ def main(argv=None):
         return xm[s], zm[s], phi[s]
 
     namp = 2.0 ** nqb
-    nat.init()
-    from phase2_amd import costmodel
-    schedule = args.schedule
-    if schedule == "auto":
-        schedule = "learned" if costmodel.PARAMS_PATH.is_file() else "greedy"
-    if schedule == "learned":
-        params, _ = costmodel.load()
-        costmodel.install(params, low=args.low, min_qb=min(nqb, 28))
     log(f"bench: allocating {nqb}-qubit register ({16 * namp / 2**30:.0f} GiB)")
     reg = nat.Register(nqb, nat.FUSED, args.tile, args.low)
     reg.prep_multidet(w["det_idx"], w["det_cf"])
@@ -208,57 +284,326 @@ def main(argv=None):
         nupd += len(bx[0]) * namp
         log(f"bench: step {k + 1}/{args.steps} (block {b})")
     nat.synchronize()
-    t1 = time.perf_counter()
+    elapsed = time.perf_counter() - t0
     prof = nat.prof_read(reset=True)
-    elapsed = t1 - t0
-    value = nupd / elapsed
-    fz = prof["fused"]
-    fused_ms = fz["ms"] / max(1, fz["launches"])
-    fused_gbs = BYTES_PER_AMP_PASS * namp / (fused_ms * 1e-3) / 1e9
-    terms_per_pass = fz["terms"] / max(1, fz["launches"])
-
-    plain = None
-    if not args.no_plain:
-        reg.configure(nat.PLAIN)
-        # terms spread evenly through each block (the per-pass cost of the
-        # plain kernel does not depend on the order)
-        per = max(1, args.plain_terms // len(blocks))
-        nt = 0
-        nat.synchronize()
-        p0 = time.perf_counter()
-        for b in blocks:
-            bx, bz, bp = block(b)
-            sel = np.linspace(0, len(bx) - 1, per).astype(int)
-            reg.rotate(bx[sel], bz[sel], bp[sel])
-            nt += len(sel)
-        nat.synchronize()
-        p1 = time.perf_counter()
-        pp = nat.prof_read(reset=True)
-        launches = pp["plain_pair"]["launches"] + pp["plain_diag"]["launches"]
-        pms = (pp["plain_pair"]["ms"] + pp["plain_diag"]["ms"]) / max(1, launches)
-        pgbs = BYTES_PER_AMP_PASS * namp / (pms * 1e-3) / 1e9
-        plain = dict(amp_updates_per_s=nt * namp / (p1 - p0),
-                     kernel_ms_per_pass=pms, kernel_GBps=pgbs,
-                     hbm_frac=pgbs / HBM_PEAK_GBS, terms=nt,
-                     pair_launches=pp["plain_pair"]["launches"],
-                     diag_launches=pp["plain_diag"]["launches"])
-        reg.configure(nat.FUSED, args.tile, args.low)
     nat.prof_enable(False)
-
     norm2 = reg.norm2()
     reg.free()
+    value = nupd / elapsed
+    fz = prof["fused"]
+    plain = None
+    if not args.no_plain:
+        # terms spread evenly through each timed block (the per-pass cost of
+        # the plain kernel does not depend on the order)
+        per = max(1, args.plain_terms // len(blocks))
+        sel = np.concatenate([b * 1024 + spread(len(block(b)[0]), per)
+                              for b in blocks])
+        plain = plain_rate(nat, nqb, xm[sel], zm[sel], phi[sel])
+    return dict(
+        value=value, elapsed=elapsed, norm2=norm2, plain=plain,
+        roofline=roofline(f"k_rot_fused<{args.tile}>", fz),
+        config={"workload": f"trott1_n{nqb}_lex_sweep_blocks", "nqb": nqb,
+                "terms": nterms, "terms_per_step": 1024,
+                "blocks_timed": blocks, "blocks_in_sweep": nblocks,
+                "delta": w["delta"]},
+        extra={"terms_per_pass": fz["terms"] / max(1, fz["launches"]),
+               "first_visit_seconds": sched_s if schedule == "learned"
+               else None},
+        cpu=lambda ranks: cpu_n33_proxy(args, nat, ranks),
+        nqb=nqb)
+
+
+def cpu_n33_proxy(args, nat, ranks):
+    """A 33-qubit register does not fit twice in host memory: the reference
+    runs `cpu_terms` lex-ordered terms of the same generator (from the middle
+    of the sweep) at `cpu_nqb` qubits."""
+    from phase2_amd import synth
+    n = args.cpu_nqb
+    hm = synth.hamiltonian(n, 0, seed=0, decay=True,
+                           n_total=n + n * (n - 1) // 2 + 1 + 4000)
+    xm, zm, cf = lex(hm, nat)
+    s = slice(len(cf) // 2, len(cf) // 2 + args.cpu_terms)
+    return cpu_rotations(n, xm[s], zm[s], 0.02 * cf[s], ranks,
+                         "lex-ordered terms of the same generator")
+
+
+def _timed_driver(nat, run, steps, warmup):
+    """run(steps) once with `warmup` steps untimed, then timed with
+    profiling on; returns (elapsed, prof)."""
+    if warmup:
+        run(warmup)
+    nat.synchronize()
+    nat.prof_enable(True)
+    nat.prof_read(reset=True)
+    t0 = time.perf_counter()
+    run(steps)
+    nat.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = nat.prof_read(reset=True)
+    nat.prof_enable(False)
+    return elapsed, prof
+
+
+def _dominant(prof, tile):
+    kinds = {"fused": f"k_rot_fused<{tile}>", "plain_pair": "k_rot_pair",
+             "plain_diag": "k_rot_diag", "batched": f"k_rot_fused_batch<{tile}>"}
+    k = max(kinds, key=lambda n: prof[n]["ms"])
+    return kinds[k], prof[k]
+
+
+def bench_trott_n16(args, nat, costmodel, schedule):
+    from phase2_amd import synth
+    w = synth.trott_n16()
+    hm = w["hm"]
+    prep = dict(det_idx=w["det_idx"], det_cf=w["det_cf"])
+    nat.set_default(nat.FUSED, args.tile, args.low)
+
+    def run(k):
+        return nat.trott(1, hm, prep, w["delta"], k, want_state=False)
+
+    # profiling adds an event pair per launch, which matters for microsecond
+    # launches: time without it, then profile a separate run
+    if args.warmup:
+        run(args.warmup)
+    nat.synchronize()
+    t0 = time.perf_counter()
+    out, _ = run(args.steps)
+    nat.synchronize()
+    elapsed = time.perf_counter() - t0
+    _, prof = _timed_driver(nat, run, args.steps, 0)
+    kern, kp = _dominant(prof, args.tile)
+    nat.set_default(nat.PLAIN)
+    t1 = time.perf_counter()
+    run(args.steps)
+    nat.synchronize()
+    tp = time.perf_counter() - t1
+    nat.set_default(nat.FUSED, args.tile, args.low)
+    nupd = args.steps * len(hm) * 2.0 ** hm.nqb
+    xm, zm, cf = lex(hm, nat)
+    return dict(
+        value=nupd / elapsed, elapsed=elapsed, norm2=None,
+        plain={"amp_updates_per_s": nupd / tp, "whole_run_seconds": tp},
+        roofline=roofline(kern, kp),
+        config={"workload": "trott1_n16_full", "nqb": 16, "terms": len(hm),
+                "steps": args.steps, "delta": w["delta"]},
+        extra={"overlap_last": [float(out[-1].real), float(out[-1].imag)]},
+        cpu=lambda ranks: cpu_circ(
+            1, synth.trott_n16()["hm"], w["det_idx"], w["det_cf"], ranks,
+            f"trott n=16, {args.steps} steps of {len(hm)} terms",
+            steps=args.steps, delta=w["delta"]),
+        nqb=16)
+
+
+def bench_trott2_n26(args, nat, costmodel, schedule):
+    from phase2_amd import synth
+    w = synth.trott2_n26()
+    hm = w["hm"]
+    prep = dict(C_alpha=w["C_alpha"], n_sites=w["n_sites"],
+                n_alpha=w["n_alpha"], n_beta=w["n_beta"],
+                closed_shell=w["closed_shell"], tapered=w["tapered"])
+    nat.set_default(nat.FUSED, args.tile, args.low)
+
+    def run(k):
+        return nat.trott(2, hm, prep, w["delta"], k, want_state=False)
+
+    elapsed, prof = _timed_driver(nat, run, args.steps, args.warmup)
+    kern, kp = _dominant(prof, args.tile)
+    nupd = args.steps * 2 * len(hm) * 2.0 ** hm.nqb
+    xm, zm, cf = lex(hm, nat)
+    sel = spread(len(cf), args.plain_terms)
+    plain = None if args.no_plain else plain_rate(
+        nat, hm.nqb, xm[sel], zm[sel], 0.5 * w["delta"] * cf[sel])
+    return dict(
+        value=nupd / elapsed, elapsed=elapsed, norm2=None, plain=plain,
+        roofline=roofline(kern, kp),
+        config={"workload": "trott2_n26_coeff_matrix", "nqb": 26,
+                "terms": len(hm), "steps": args.steps, "delta": w["delta"],
+                "determinants": 230230},
+        extra={"terms_per_pass": kp["terms"] / max(1, kp["launches"])},
+        cpu=lambda ranks: cpu_rotations(
+            26, xm[:128], zm[:128], 0.5 * w["delta"] * cf[:128], ranks,
+            "first 128 rotations of a trott2 forward half-sweep"),
+        nqb=26)
+
+
+def bench_cmpsit_n30(args, nat, costmodel, schedule):
+    from phase2_amd import synth
+    w = synth.cmpsit_n30()
+    hm = w["hm"]
+    prep = dict(det_idx=w["det_idx"], det_cf=w["det_cf"])
+    nat.set_default(nat.FUSED, args.tile, args.low)
+
+    def run(k):
+        return nat.cmpsit(hm, prep, 1, k, w["length"], w["depth"],
+                          w["angle_det"], w["angle_rand"], w["seed"],
+                          want_state=False)
+
+    elapsed, prof = _timed_driver(nat, run, args.steps, args.warmup)
+    kern, kp = _dominant(prof, args.tile)
+    rot_per_step = 2 * (w["length"] + w["depth"])
+    nupd = args.steps * rot_per_step * 2.0 ** hm.nqb
+    det = np.argsort(-np.abs(hm.cf), kind="stable")[:w["length"]]
+    dx, dz, dc = hm.xm[det], hm.zm[det], hm.cf[det]
+    perm = nat.sort_lex(dx, dz)
+    dx, dz, dc = dx[perm], dz[perm], dc[perm]
+    sel = spread(len(dc), args.plain_terms)
+    phi = 0.5 * w["angle_det"] * dc
+    plain = None if args.no_plain else plain_rate(
+        nat, hm.nqb, dx[sel], dz[sel], phi[sel])
+    return dict(
+        value=nupd / elapsed, elapsed=elapsed, norm2=None, plain=plain,
+        roofline=roofline(kern, kp),
+        config={"workload": "cmpsit_n30", "nqb": 30, "terms": len(hm),
+                "length": w["length"], "depth": w["depth"],
+                "steps": args.steps, "rotations_per_step": rot_per_step,
+                "angle_det": w["angle_det"], "angle_rand": w["angle_rand"]},
+        extra={"terms_per_pass": kp["terms"] / max(1, kp["launches"])},
+        cpu=lambda ranks: cpu_rotations(
+            30, dx[:16], dz[:16], phi[:16], ranks,
+            "first 16 deterministic rotations of a cmpsit half-step"),
+        nqb=30)
+
+
+def bench_qdrift_batch(args, nat, costmodel, schedule):
+    from phase2_amd import synth
+    regs = synth.qdrift_batch(args.regs, samples=args.steps)
+    # warm-up: a few registers, one sample
+    warm = synth.qdrift_batch(16, seed=1, samples=1)
+    if args.warmup:
+        nat.batch_qdrift(warm, nat.BATCH_FUSED_BATCHED, args.tile, args.low)
+    nat.synchronize()
+    t0 = time.perf_counter()
+    _, st = nat.batch_qdrift(regs, nat.BATCH_FUSED_BATCHED, 0, args.low)
+    nat.synchronize()
+    elapsed = time.perf_counter() - t0
+    # kernel timing on a subset, without graph capture (profiling brackets
+    # every launch with events)
+    sub = [dict(r, samples=1) for r in regs[:64]]
+    nat.prof_enable(True)
+    nat.prof_read(reset=True)
+    nat.batch_qdrift(sub, nat.BATCH_FUSED_BATCHED, st["tile_bits"],
+                     st["low_bits"])
+    prof = nat.prof_read(reset=True)
+    nat.prof_enable(False)
+    _, sp = nat.batch_qdrift(sub, nat.BATCH_PLAIN_EACH, 12, args.low)
+    _, sf = nat.batch_qdrift(sub, nat.BATCH_FUSED_EACH, st["tile_bits"],
+                             st["low_bits"])
+    nsamp = sum(r["samples"] for r in regs)
+    widths = [r["hm"].nqb for r in regs]
+    return dict(
+        value=st["amp_updates"] / elapsed, elapsed=elapsed, norm2=None,
+        plain={"amp_updates_per_s": sp["amp_updates"] / sp["seconds"],
+               "subset": "first 64 registers, 1 sample, one launch per "
+               "rotation", "launches": sp["launches"]},
+        roofline=roofline(f"k_rot_fused_batch<{st['tile_bits']}>",
+                          prof["batched"]),
+        config={"workload": "qdrift_batch_512", "registers": len(regs),
+                "widths": [int(min(widths)), int(max(widths))],
+                "samples": args.steps, "depth": 1000,
+                "buffer_GiB": sum(2.0 ** n for n in widths) * 16 / 2 ** 30},
+        extra={"samples_per_s": nsamp / elapsed,
+               "launches": st["launches"], "passes": st["passes"],
+               "tile_bits": st["tile_bits"], "low_bits": st["low_bits"],
+               "tune_seconds": st["tune_seconds"],
+               "fused_each_amp_updates_per_s":
+                   sf["amp_updates"] / sf["seconds"]},
+        cpu=lambda ranks: cpu_qdrift_sample(regs, ranks),
+        nqb=None)
+
+
+def cpu_qdrift_sample(regs, ranks):
+    """Reference qdrift_simul on the first registers of 20..22 qubits of
+    the batch, one sample each."""
+    pick = [r for r in regs if 20 <= r["hm"].nqb <= 22][:3]
+    tot_upd = tot_s = 0.0
+    res = None
+    for r in pick:
+        res = cpu_circ(3, r["hm"], r["det_idx"], r["det_cf"], ranks, "",
+                       samples=1, depth=r["depth"], delta=r["step_size"],
+                       seed=r["seed"])
+        if res is None:
+            return None
+        tot_upd += res["value"] * res["seconds"]
+        tot_s += res["seconds"]
+    if not pick or tot_s == 0:
+        return None
+    widths = ",".join(str(r["hm"].nqb) for r in pick)
+    return dict(value=tot_upd / tot_s, unit="amp-updates/s",
+                cores=res["cores"], kind="reference",
+                sample=f"qdrift_simul, 1 sample x 1000 draws on {len(pick)} "
+                f"registers of the batch (widths {widths}), "
+                f"{res['cores']} MPI ranks x 1 thread each",
+                seconds=tot_s)
+
+
+BENCH = dict(trott_n33=bench_trott_n33, trott_n16=bench_trott_n16,
+             trott2_n26=bench_trott2_n26, qdrift_batch=bench_qdrift_batch,
+             cmpsit_n30=bench_cmpsit_n30)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=WORKLOADS, default="trott_n33")
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nqb", type=int, default=33,
+                    help="register width of trott_n33 (smaller for quick runs)")
+    ap.add_argument("--regs", type=int, default=512,
+                    help="registers of qdrift_batch")
+    ap.add_argument("--tile", type=int, default=12)
+    ap.add_argument("--low", type=int, default=4)
+    ap.add_argument("--plain-terms", type=int, default=24)
+    ap.add_argument("--schedule", choices=("auto", "greedy", "learned"),
+                    default="auto",
+                    help="fusion schedule; auto = learned when the trained "
+                    "parameters (phase2_amd/costmodel_params.npz) exist")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--cpu-baseline", dest="cpu", action="store_true",
+                    default=True)
+    ap.add_argument("--no-cpu-baseline", dest="cpu", action="store_false")
+    ap.add_argument("--cpu-nqb", type=int, default=28)
+    ap.add_argument("--cpu-terms", type=int, default=16)
+    ap.add_argument("--cpu-ranks", type=int, default=0)
+    args = ap.parse_args(argv)
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, "
+              f"WORLD_SIZE={world}): the register lives on a single MI355X "
+              "and there is no multi-GPU path in this build.",
+              file=sys.stderr, flush=True)
+        sys.exit(2)
+    if args.steps is None:
+        args.steps = DEFAULT_STEPS[args.workload]
+
+    from phase2_amd import _native as nat
+    from phase2_amd import costmodel
+    nat.init()
+    schedule = args.schedule
+    if schedule == "auto":
+        schedule = "learned" if costmodel.PARAMS_PATH.is_file() else "greedy"
+    if schedule == "learned":
+        params, _ = costmodel.load()
+        costmodel.install(params, low=args.low, min_qb=28)
+
+    r = BENCH[args.workload](args, nat, costmodel, schedule)
     if schedule == "learned":
         costmodel.uninstall()
+    roof = r["roofline"]
+    if r["nqb"] is not None:
+        roof["traffic"] = traffic_for(roof["kernel"], r["nqb"])
 
     cpu = None
     if args.cpu:
-        ranks = args.cpu_ranks
-        if ranks <= 0:
-            ncpu = min(os.cpu_count() or 1, 16)
-            ranks = 1 << int(math.log2(ncpu))
-        log(f"bench: CPU baseline, {ranks} ranks, nqb={args.cpu_nqb}")
-        cpu = cpu_baseline(args.cpu_nqb, args.cpu_terms, ranks)
+        ranks = cpu_ranks(args)
+        log(f"bench: CPU baseline, {ranks} ranks")
+        cpu = r["cpu"](ranks)
 
+    value = r["value"]
+    plain = r["plain"]
+    cfg = dict(r["config"])
+    cfg.update(schedule=schedule, tile_bits=args.tile, low_bits=args.low)
     out = {
         "metric": METRIC,
         "value": value,
@@ -266,45 +611,22 @@ def main(argv=None):
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
+        "ms_per_step": 1e3 * r["elapsed"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {
-            "workload": f"trott1_n{nqb}_lex_sweep_blocks",
-            "nqb": nqb,
-            "terms": nterms,
-            "terms_per_step": 1024,
-            "blocks_timed": blocks,
-            "blocks_in_sweep": nblocks,
-            "delta": w["delta"],
-            "schedule": schedule,
-            "tile_bits": args.tile,
-            "low_bits": args.low,
-        },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": f"k_rot_fused<{args.tile}>",
-            "achieved": fused_gbs,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": fused_gbs / HBM_PEAK_GBS,
-            "traffic": None,
-            "algorithmic_bytes_per_launch": BYTES_PER_AMP_PASS * namp,
-            "mean_launch_ms": fused_ms,
-            "launches": fz["launches"],
-        },
-        "fused": {
-            "amp_updates_per_s": value,
-            "terms_per_pass": terms_per_pass,
-            "effective_GBps": BYTES_PER_AMP_PASS * value / 1e9,
-        },
+        "config": cfg,
+        "roofline": roof,
+        "fused": dict(amp_updates_per_s=value,
+                      effective_GBps=BYTES_PER_AMP_PASS * value / 1e9,
+                      **r["extra"]),
         "plain": plain,
-        "fused_vs_plain": (value / plain["amp_updates_per_s"]) if plain else None,
-        "norm_drift": abs(1.0 - norm2),
-        "first_visit_seconds": sched_s if schedule == "learned" else None,
+        "fused_vs_plain": (value / plain["amp_updates_per_s"])
+        if plain else None,
+        "norm_drift": abs(1.0 - r["norm2"]) if r["norm2"] is not None
+        else None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

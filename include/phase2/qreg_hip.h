@@ -13,7 +13,7 @@
  *
  * Defaults come from PHASE2_HIP_MODE (plain|fused), PHASE2_HIP_TILE (tile
  * bits, 9..13) and PHASE2_HIP_LOW (contiguous low bits); otherwise fused,
- * 12 and 6.  Registers narrower than 9 qubits always use the plain sweep.
+ * 12 and 4.  Registers narrower than 9 qubits always use the plain sweep.
  */
 
 #include <stddef.h>

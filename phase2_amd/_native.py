@@ -121,7 +121,7 @@ def init():
         _err("ph2_init")
 
 
-def set_default(mode=FUSED, tile_bits=12, low_bits=6):
+def set_default(mode=FUSED, tile_bits=12, low_bits=4):
     lib().ph2_set_default(int(mode), int(tile_bits), int(low_bits))
 
 
@@ -175,7 +175,7 @@ def cmpsit(hm, prep_kw, samples, steps, length, depth, angle_det,
     return out, st
 
 
-def rotate(state, xm, zm, phi, mode=FUSED, tile_bits=12, low_bits=6):
+def rotate(state, xm, zm, phi, mode=FUSED, tile_bits=12, low_bits=4):
     st = np.array(state, dtype=np.complex128, copy=True)
     nqb = int(st.shape[0]).bit_length() - 1
     xm = np.ascontiguousarray(xm, dtype=np.uint64)
@@ -241,7 +241,7 @@ class Prof(C.Structure):
     """struct ph2hip_prof: kinds 0 plain pair, 1 plain diagonal, 2 fused,
     3 batched fused."""
     _fields_ = [("launches", C.c_uint64 * 4), ("ms", C.c_double * 4),
-                ("terms", C.c_uint64 * 4)]
+                ("terms", C.c_uint64 * 4), ("amps", C.c_uint64 * 4)]
 
 
 def prof_enable(on: bool):
@@ -255,7 +255,8 @@ def prof_read(reset=True) -> dict:
         _err("ph2hip_prof_read")
     kinds = ("plain_pair", "plain_diag", "fused", "batched")
     return {k: dict(launches=int(p.launches[i]), ms=float(p.ms[i]),
-                    terms=int(p.terms[i])) for i, k in enumerate(kinds)}
+                    terms=int(p.terms[i]), amps=int(p.amps[i]))
+            for i, k in enumerate(kinds)}
 
 
 def synchronize():
@@ -266,7 +267,7 @@ def synchronize():
 class Register:
     """A device-resident register (struct qreg behind ph2_reg_*)."""
 
-    def __init__(self, nqb, mode=FUSED, tile_bits=12, low_bits=6):
+    def __init__(self, nqb, mode=FUSED, tile_bits=12, low_bits=4):
         L = lib()
         L.ph2_reg_create.restype = C.c_void_p
         for f in ("ph2_reg_prep_multidet", "ph2_reg_rotate",
@@ -279,7 +280,7 @@ class Register:
         if not self.h:
             _err(f"ph2_reg_create({nqb})")
 
-    def configure(self, mode, tile_bits=12, low_bits=6):
+    def configure(self, mode, tile_bits=12, low_bits=4):
         lib().ph2_reg_configure(C.c_void_p(self.h), C.c_int(mode),
                                 C.c_uint32(tile_bits), C.c_uint32(low_bits))
 
@@ -358,6 +359,15 @@ def set_fused_form(pipe_mask: int, persistent_wgs: int = 0):
     if lib().ph2hip_set_fused_form(C.c_uint32(pipe_mask),
                                    C.c_uint64(persistent_wgs)) != 0:
         _err("ph2hip_set_fused_form")
+
+
+def set_fused_dma(dma_mask: int, wgs: int = 0):
+    """Bit TB of dma_mask: double-buffered LDS-DMA fused kernel for tile TB
+    (wgs workgroups; 0 = all resident ones)."""
+    L = lib()
+    L.ph2hip_set_fused_dma.restype = C.c_int
+    if L.ph2hip_set_fused_dma(C.c_uint32(dma_mask), C.c_uint64(wgs)) != 0:
+        _err("ph2hip_set_fused_dma")
 
 
 def set_plain_form(form: int):

@@ -43,6 +43,7 @@ char g_err[256] = "";
 hipStream_t g_stream = nullptr;
 int g_device = -1;
 uint64_t g_persist_wgs = 512;   // fused-pass workgroups (2 per CU by default)
+uint32_t g_ncu = 256;          // compute units of the device
 double *g_red = nullptr;       // reduction partials (device)
 size_t g_red_cap = 0;
 
@@ -228,21 +229,46 @@ __global__ __launch_bounds__(kBlock) void k_rot_diag(
 // LDS slots are XOR-swizzled (bits >= 4 folded into bits 0..3) so that the
 // strided cosets of a wave spread over the banks.
 
-struct FusedTerm {
+// The descriptors are kernel arguments read with wave-uniform indices.
+// Every field the kernels read is an aligned 32-bit word (byte fields are
+// read through the word that holds them), so all of them come in through
+// scalar loads: a byte or unaligned read would become a vector-memory load
+// that shares the wait counter with the tile's HBM traffic.
+struct alignas(16) FusedTerm {
 	uint64_t z;    // physical phase mask (per-tile parity)
 	double c, s;
 	uint32_t zt;   // phase mask in tile coordinates
-	uint8_t xtp;   // flip mask in group coordinates
-	uint8_t ztp;   // parity(u_i & zt), group coordinates
-	uint8_t y;     // popcount(x & z) & 3
-	uint8_t pad;
+	union {
+		struct {
+			uint8_t xtp;   // flip mask in group coordinates
+			uint8_t ztp;   // parity(u_i & zt), group coordinates
+			uint8_t y;     // popcount(x & z) & 3
+			uint8_t pad;
+		};
+		uint32_t meta;
+	};
 };
 
-struct FusedGroup {
-	uint16_t u[4];   // group basis, tile coordinates
-	uint8_t piv[4];  // its leading bits, ascending
-	uint8_t jpos[12];// tile bit of each coset-index bit (the free bits)
-	uint8_t first, count, pad[2];
+struct alignas(16) FusedGroup {
+	union {
+		uint16_t u[4];   // group basis, tile coordinates
+		uint32_t uw[2];
+	};
+	union {
+		uint8_t piv[4];  // its leading bits, ascending
+		uint32_t pivw;
+	};
+	union {
+		uint8_t jpos[12];// tile bit of each coset-index bit (the free bits)
+		uint32_t jposw[3];
+	};
+	union {
+		struct {
+			uint8_t first, count, pad2[2];
+		};
+		uint32_t fcw;
+	};
+	uint32_t pad3[1];
 };
 
 constexpr int kMaxPer = (1 << PH2HIP_TILE_MAX) / kBlock;
@@ -250,11 +276,30 @@ constexpr int kMaxPer = (1 << PH2HIP_TILE_MAX) / kBlock;
 struct FusedPass {
 	uint64_t w[PH2HIP_TILE_MAX];   // gathered basis vectors
 	uint64_t roff[kMaxPer];        // gathered offset of load/store step m
-	uint8_t piv[PH2HIP_TILE_MAX];  // their leading bits, ascending
-	uint8_t L, g, ngroups, nterms;
+	uint32_t piv[PH2HIP_TILE_MAX]; // their leading bits, ascending
+	union {
+		struct {
+			uint8_t L, g, ngroups, nterms;
+		};
+		uint32_t hdr;
+	};
 	FusedGroup grp[PH2HIP_PASS_MAX];
 	FusedTerm t[PH2HIP_PASS_MAX];
 };
+
+// Device-side reads of the packed fields.
+__device__ __forceinline__ uint32_t pass_L(const FusedPass &P)
+{
+	return P.hdr & 0xffu;
+}
+__device__ __forceinline__ uint32_t pass_g(const FusedPass &P)
+{
+	return (P.hdr >> 8) & 0xffu;
+}
+__device__ __forceinline__ uint32_t pass_ngroups(const FusedPass &P)
+{
+	return (P.hdr >> 16) & 0xffu;
+}
 
 static_assert(sizeof(FusedPass) <= 4000, "kernel argument too large");
 
@@ -323,8 +368,9 @@ __device__ __forceinline__ void coset_dispatch(double (&re)[1 << D],
 
 __device__ __forceinline__ uint64_t tile_base(const FusedPass &P, uint64_t t)
 {
-	uint64_t base = t << P.L;
-	for (uint32_t b = 0; b < P.g; b++)
+	uint64_t base = t << pass_L(P);
+	const uint32_t g = pass_g(P);
+	for (uint32_t b = 0; b < g; b++)
 		base = insert_zero(base, P.piv[b]);
 	return base;
 }
@@ -334,21 +380,50 @@ __device__ __forceinline__ uint64_t tile_base(const FusedPass &P, uint64_t t)
 // registers while the current tile is rotated in LDS, and its writes are
 // issued without waiting.
 // All micro-groups of a pass on one tile held in LDS.
-template <int TB, int D>
+// Workgroup barrier.  LDSONLY: orders LDS accesses only (waits for the
+// wave's LDS operations, not for its outstanding global loads and stores),
+// so HBM traffic issued before it stays in flight.
+template <bool LDSONLY>
+__device__ __forceinline__ void tile_barrier()
+{
+	if constexpr (LDSONLY) {
+		// lgkmcnt(0) (this wave's LDS accesses are done; vmcnt and expcnt
+		// at their maxima, i.e. not waited for), then the barrier.  A
+		// workgroup fence would also wait for the LDS-writing HBM loads.
+		// The empty asm statements keep the compiler from moving any
+		// memory access across.
+		__asm__ volatile("" ::: "memory");
+		__builtin_amdgcn_s_waitcnt(0xC07F);
+		__builtin_amdgcn_s_barrier();
+		__asm__ volatile("" ::: "memory");
+	} else {
+		__syncthreads();
+	}
+}
+
+template <int TB, int D, bool LDSONLY = false>
 __device__ __forceinline__ void tile_groups(v2d *tile, const FusedPass &P,
 	uint64_t base, uint32_t tid)
 {
 	constexpr uint32_t NSUB = (1u << TB) >> D;    // cosets per group
 	static_assert(NSUB % kBlock == 0, "cosets must cover the block");
-	for (uint32_t gi = 0; gi < P.ngroups; gi++) {
-		const FusedGroup G = P.grp[gi];
-		__syncthreads();
+	const uint32_t ngroups = pass_ngroups(P);
+	for (uint32_t gi = 0; gi < ngroups; gi++) {
+		const uint32_t uw0 = P.grp[gi].uw[0], uw1 = P.grp[gi].uw[1];
+		const uint32_t jw0 = P.grp[gi].jposw[0], jw1 = P.grp[gi].jposw[1],
+			jw2 = P.grp[gi].jposw[2];
+		const uint32_t fcw = P.grp[gi].fcw;
+		const uint32_t gu[4] = { uw0 & 0xffffu, uw0 >> 16, uw1 & 0xffffu,
+			uw1 >> 16 };
+		const uint32_t jw[3] = { jw0, jw1, jw2 };
+		tile_barrier<LDSONLY>();
 		for (uint32_t j = tid; j < NSUB; j += kBlock) {
 			// coset base: j's bits spread over the free tile bits
 			uint32_t bu = 0;
 #pragma unroll
 			for (int k = 0; k < TB - D; k++)
-				bu |= (j >> k & 1u) << G.jpos[k];
+				bu |= (j >> k & 1u) <<
+					((jw[k >> 2] >> (8 * (k & 3))) & 0xffu);
 			const uint32_t sbu = lds_slot(bu);
 			double re[1 << D], im[1 << D];
 #pragma unroll
@@ -357,24 +432,29 @@ __device__ __forceinline__ void tile_groups(v2d *tile, const FusedPass &P,
 #pragma unroll
 				for (int i = 0; i < D; i++)
 					if (w >> i & 1)
-						cw ^= G.u[i];
+						cw ^= gu[i];
 				const v2d v = tile[sbu ^ lds_slot(cw)];
 				re[w] = v.x;
 				im[w] = v.y;
 			}
-			for (uint32_t k = G.first; k < (uint32_t)G.first + G.count;
-				k++) {
-				const FusedTerm T = P.t[k];
-				const uint32_t bpt = (__popc(bu & T.zt) ^
-					__popcll(base & T.z)) & 1u;
-				const double c = T.c;
-				if (T.xtp == 0) {
+			const uint32_t kfirst = fcw & 0xffu;
+			const uint32_t kend = kfirst + ((fcw >> 8) & 0xffu);
+			for (uint32_t k = kfirst; k < kend; k++) {
+				const uint64_t tz = P.t[k].z;
+				const double c = P.t[k].c, ts = P.t[k].s;
+				const uint32_t tzt = P.t[k].zt, meta = P.t[k].meta;
+				const uint32_t xtp = meta & 0xffu;
+				const uint32_t ztp = (meta >> 8) & 0xffu;
+				const uint32_t ty = (meta >> 16) & 0xffu;
+				const uint32_t bpt = (__popc(bu & tzt) ^
+					__popcll(base & tz)) & 1u;
+				if (xtp == 0) {
 					// diagonal: a <- c a + s_j (i a), s_j = +-s
-					const double sb = flip_sign(T.s, bpt);
+					const double sb = flip_sign(ts, bpt);
 #pragma unroll
 					for (int w = 0; w < (1 << D); w++) {
 						const double sj = flip_sign(sb,
-							__popc(w & T.ztp) & 1u);
+							__popc(w & ztp) & 1u);
 						const double nr = add(mul(c, re[w]),
 							mul(sj, -im[w]));
 						const double ni = add(mul(c, im[w]),
@@ -383,14 +463,14 @@ __device__ __forceinline__ void tile_groups(v2d *tile, const FusedPass &P,
 						im[w] = ni;
 					}
 				} else {
-					const double sb = flip_sign(T.s,
-						bpt ^ ((T.y >> 1) & 1u));
-					if (T.y & 1)
-						coset_dispatch<D, true>(re, im, T.xtp,
-							T.ztp, sb, c);
+					const double sb = flip_sign(ts,
+						bpt ^ ((ty >> 1) & 1u));
+					if (ty & 1)
+						coset_dispatch<D, true>(re, im, xtp,
+							ztp, sb, c);
 					else
-						coset_dispatch<D, false>(re, im, T.xtp,
-							T.ztp, sb, c);
+						coset_dispatch<D, false>(re, im, xtp,
+							ztp, sb, c);
 				}
 			}
 #pragma unroll
@@ -399,7 +479,7 @@ __device__ __forceinline__ void tile_groups(v2d *tile, const FusedPass &P,
 #pragma unroll
 				for (int i = 0; i < D; i++)
 					if (w >> i & 1)
-						cw ^= G.u[i];
+						cw ^= gu[i];
 				v2d v;
 				v.x = re[w];
 				v.y = im[w];
@@ -407,7 +487,7 @@ __device__ __forceinline__ void tile_groups(v2d *tile, const FusedPass &P,
 			}
 		}
 	}
-	__syncthreads();
+	tile_barrier<LDSONLY>();
 }
 
 // Load one tile from HBM into LDS, apply the pass, write it back.
@@ -416,9 +496,10 @@ __device__ __forceinline__ void fused_tile(v2d *__restrict__ a,
 	v2d *tile, const FusedPass &P, uint64_t t)
 {
 	constexpr uint32_t PER = (1u << TB) / kBlock;
-	const uint32_t L = P.L, lmask = (1u << L) - 1, tid = threadIdx.x;
+	const uint32_t L = pass_L(P), lmask = (1u << L) - 1, tid = threadIdx.x;
+	const uint32_t g = pass_g(P);
 	uint64_t wt = 0;   // gathered offset of this thread's row part
-	for (uint32_t b = 0; b < P.g; b++)
+	for (uint32_t b = 0; b < g; b++)
 		if ((tid >> L) >> b & 1)
 			wt ^= P.w[b];
 	const uint64_t base = tile_base(P, t);
@@ -490,9 +571,10 @@ __global__ __launch_bounds__(kBlock) void k_rot_fused_pipe(
 	constexpr uint32_t PER = (1u << TB) / kBlock;
 	__shared__ v2d tile[1u << TB];
 	v2d *__restrict__ a = reinterpret_cast<v2d *>(amp);
-	const uint32_t L = P.L, lmask = (1u << L) - 1, tid = threadIdx.x;
+	const uint32_t L = pass_L(P), lmask = (1u << L) - 1, tid = threadIdx.x;
+	const uint32_t g = pass_g(P);
 	uint64_t wt = 0;
-	for (uint32_t b = 0; b < P.g; b++)
+	for (uint32_t b = 0; b < g; b++)
 		if ((tid >> L) >> b & 1)
 			wt ^= P.w[b];
 	v2d pre[PER];
@@ -523,6 +605,116 @@ __global__ __launch_bounds__(kBlock) void k_rot_fused_pipe(
 		for (uint32_t m = 0; m < PER; m++)
 			a[bo ^ P.roff[m] ^ ((tid + m * kBlock) & lmask)] =
 				tile[lds_slot(tid + m * kBlock)];
+	}
+}
+
+// Double-buffered persistent pass with direct HBM->LDS loads (gfx950
+// global_load_lds_dwordx4): while a workgroup rotates tile t in one LDS
+// buffer, the loads of its next tile land in the other without passing
+// through registers, and the write-back of t drains behind them.  Lane l of
+// wave w in load step m fills LDS slot s = m*256 + 64w + l, i.e. tile
+// coordinate lds_slot(s) = s ^ f with f = ((s >> 4) ^ (s >> 8)) & 15 (the
+// swizzle is an involution).  f only touches coordinate bits 0..3, which
+// lie inside a row when L >= 4, so the lane reads column (s ^ f) & lmask of
+// the row of s: every 16 lanes still cover one 256-byte run.  Requires
+// L >= 4 (checked on the host).
+#define PH2_LDS(p) ((__attribute__((address_space(3))) void *)(p))
+
+// s_waitcnt with vmcnt <= N and no wait on the other counters (gfx9
+// encoding: vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8).
+template <uint32_t N>
+__device__ __forceinline__ void vm_wait()
+{
+	static_assert(N < 64, "vmcnt field");
+	__asm__ volatile("" ::: "memory");
+	__builtin_amdgcn_s_waitcnt((N & 15u) | ((N >> 4) << 14) | (7u << 4) |
+		(15u << 8));
+	__asm__ volatile("" ::: "memory");
+}
+
+template <int TB>
+__device__ __forceinline__ void dma_tile(const v2d *__restrict__ a,
+	v2d *__restrict__ buf,
+	const FusedPass &P, uint64_t bo, uint32_t tid)
+{
+	constexpr uint32_t PER = (1u << TB) / kBlock;
+	const uint32_t lmask = (1u << pass_L(P)) - 1;
+	const uint32_t wave0 = __builtin_amdgcn_readfirstlane(tid & ~63u);
+#pragma unroll
+	for (uint32_t m = 0; m < PER; m++) {
+		const uint32_t u = tid + m * kBlock;
+		const uint32_t f = ((tid >> 4) ^ m) & 15u;
+		__builtin_amdgcn_global_load_lds(
+			(const void *)(a + (bo ^ P.roff[m] ^ ((u ^ f) & lmask))),
+			PH2_LDS(buf + m * kBlock + wave0), 16, 0, 0);
+	}
+}
+
+template <int TB>
+__device__ __forceinline__ void store_tile(v2d *__restrict__ a,
+	const v2d *buf, const FusedPass &P, uint64_t bo, uint32_t tid)
+{
+	constexpr uint32_t PER = (1u << TB) / kBlock;
+	const uint32_t lmask = (1u << pass_L(P)) - 1;
+#pragma unroll
+	for (uint32_t m = 0; m < PER; m++)
+		a[bo ^ P.roff[m] ^ ((tid + m * kBlock) & lmask)] =
+			buf[lds_slot(tid + m * kBlock)];
+}
+
+// One half of the double-buffered loop: barrier (tile t has landed in
+// `cur` in every wave; nobody reads `nxt` any more), prefetch tile tn into
+// `nxt`, rotate t in `cur`, write it back.  cur and nxt are restrict
+// parameters of one inlined call, so the compiler can tell that accesses to
+// cur do not alias the loads landing in nxt and adds no wait for them.
+template <int TB, int D>
+__device__ __forceinline__ void dma_half(v2d *__restrict__ a,
+	v2d *__restrict__ cur, v2d *__restrict__ nxt, const FusedPass &P,
+	uint64_t t, uint64_t tn, uint64_t ntiles, uint64_t wt, uint32_t tid)
+{
+	tile_barrier<true>();
+	if (tn < ntiles)
+		dma_tile<TB>(a, nxt, P, tile_base(P, tn) ^ wt, tid);
+	const uint64_t b0 = tile_base(P, t);
+	tile_groups<TB, D, true>(cur, P, b0, tid);
+	store_tile<TB>(a, cur, P, b0 ^ wt, tid);
+}
+
+template <int TB, int D>
+__global__ __launch_bounds__(kBlock) void k_rot_fused_dma(
+	double2 *__restrict__ amp, const FusedPass P, uint64_t ntiles)
+{
+	constexpr uint32_t PER = (1u << TB) / kBlock;
+	__shared__ v2d bufA[1u << TB];
+	__shared__ v2d bufB[1u << TB];
+	v2d *__restrict__ a = reinterpret_cast<v2d *>(amp);
+	const uint32_t L = pass_L(P), tid = threadIdx.x;
+	const uint32_t g = pass_g(P);
+	uint64_t wt = 0;   // gathered offset of this thread's row part
+	for (uint32_t b = 0; b < g; b++)
+		if ((tid >> L) >> b & 1)
+			wt ^= P.w[b];
+	const uint64_t G = gridDim.x;
+	uint64_t t = blockIdx.x;
+	if (t >= ntiles)
+		return;           // whole workgroup
+	dma_tile<TB>(a, bufA, P, tile_base(P, t) ^ wt, tid);
+	vm_wait<0>();
+	// After each half only that half's PER stores may still be in flight
+	// once the prefetch has landed (memory operations complete in issue
+	// order): wait for vmcnt <= PER.
+	for (;;) {
+		const uint64_t t1 = t + G;
+		dma_half<TB, D>(a, bufA, bufB, P, t, t1, ntiles, wt, tid);
+		if (t1 >= ntiles)
+			break;
+		vm_wait<PER>();
+		const uint64_t t2 = t1 + G;
+		dma_half<TB, D>(a, bufB, bufA, P, t1, t2, ntiles, wt, tid);
+		if (t2 >= ntiles)
+			break;
+		vm_wait<PER>();
+		t = t2;
 	}
 }
 
@@ -676,6 +868,7 @@ bool g_prof = false;
 hipEvent_t g_pev[kProfPool][2];
 int g_pkind[kProfPool];
 uint32_t g_pterms[kProfPool];
+uint64_t g_pamps[kProfPool];
 int g_pn = 0, g_pev_made = 0;
 ph2hip_prof g_pacc = {};
 ph2hip_launch_rec g_pmeta[kProfPool];   // shape of each pending launch
@@ -693,6 +886,7 @@ int prof_drain()
 		g_pacc.launches[g_pkind[i]]++;
 		g_pacc.ms[g_pkind[i]] += ms;
 		g_pacc.terms[g_pkind[i]] += g_pterms[i];
+		g_pacc.amps[g_pkind[i]] += g_pamps[i];
 		if (!g_rec) {
 			g_rec = (ph2hip_launch_rec *)calloc(PH2HIP_REC_MAX,
 				sizeof *g_rec);
@@ -712,7 +906,8 @@ int prof_drain()
 }
 
 // Event pair for the next launch (nullptr events when profiling is off).
-int prof_begin(int kind, uint32_t nterms, hipEvent_t *stop,
+// amps: amplitudes the launch reads and writes once each.
+int prof_begin(int kind, uint32_t nterms, uint64_t amps, hipEvent_t *stop,
 	const ph2hip_launch_rec *meta = nullptr)
 {
 	*stop = nullptr;
@@ -727,6 +922,7 @@ int prof_begin(int kind, uint32_t nterms, hipEvent_t *stop,
 	}
 	g_pkind[g_pn] = kind;
 	g_pterms[g_pn] = nterms;
+	g_pamps[g_pn] = amps;
 	if (meta)
 		g_pmeta[g_pn] = *meta;
 	else
@@ -809,6 +1005,8 @@ constexpr int group_depth(int tb)
 
 bool g_pipe_init = false;
 uint32_t g_pipe_mask = 0;   // bit TB set: persistent pipelined form
+uint32_t g_dma_mask = 0;    // bit TB set: double-buffered LDS-DMA form
+uint64_t g_dma_wgs[PH2HIP_TILE_MAX + 1];   // its grid per tile size
 
 template <int TB>
 int launch_fused(double2 *a, uint32_t nqb, const FusedPass &P,
@@ -817,6 +1015,9 @@ int launch_fused(double2 *a, uint32_t nqb, const FusedPass &P,
 	if (!g_pipe_init) {
 		const char *e = getenv("PHASE2_HIP_PIPE");
 		g_pipe_mask = e ? (uint32_t)strtoul(e, nullptr, 0) : 0;
+		const char *d = getenv("PHASE2_HIP_DMA");
+		if (d)
+			g_dma_mask = (uint32_t)strtoul(d, nullptr, 0);
 		g_pipe_init = true;
 	}
 	const uint64_t ntiles = UINT64_C(1) << (nqb - TB);
@@ -831,8 +1032,27 @@ int launch_fused(double2 *a, uint32_t nqb, const FusedPass &P,
 		meta.pivots |= UINT64_C(1) << P.piv[b];
 	meta.xunion = xunion;
 	meta.zunion = zunion;
-	if (prof_begin(2, P.nterms, &stop, &meta))
+	if (prof_begin(2, P.nterms, UINT64_C(1) << nqb, &stop, &meta))
 		return -1;
+	if constexpr (TB <= 12) {
+		if ((g_dma_mask >> TB & 1) && P.L >= 4) {
+			if (!g_dma_wgs[TB]) {
+				int per_cu = 0;
+				HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+					&per_cu, k_rot_fused_dma<TB, group_depth(TB)>,
+					kBlock, 0));
+				g_dma_wgs[TB] = (uint64_t)(per_cu > 0 ? per_cu : 1) *
+					(uint64_t)g_ncu;
+			}
+			const uint64_t grid = ntiles < g_dma_wgs[TB] ? ntiles
+				: g_dma_wgs[TB];
+			hipLaunchKernelGGL((k_rot_fused_dma<TB, group_depth(TB)>),
+				dim3((uint32_t)grid), dim3(kBlock), 0, g_stream, a, P,
+				ntiles);
+			HIPCHK(hipGetLastError());
+			return prof_end(stop);
+		}
+	}
 	if (g_pipe_mask >> TB & 1) {
 		const uint64_t grid = ntiles < g_persist_wgs ? ntiles
 			: g_persist_wgs;
@@ -1101,8 +1321,10 @@ int ph2hip_init(int device)
 	g_device = device;
 	int ncu = 0;
 	if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
-		    device) == hipSuccess && ncu > 0)
+		    device) == hipSuccess && ncu > 0) {
 		g_persist_wgs = 2 * (uint64_t)ncu;
+		g_ncu = (uint32_t)ncu;
+	}
 	const char *pw = getenv("PHASE2_HIP_FUSED_WGS");
 	if (pw && atoll(pw) > 0)
 		g_persist_wgs = (uint64_t)atoll(pw);
@@ -1219,7 +1441,8 @@ int ph2hip_rotate_plain(void *amp, uint32_t nqb, const struct ph2hip_rot *rot,
 		meta.nqb = nqb;
 		meta.xunion = x;
 		meta.zunion = z;
-		if (prof_begin(x == 0 ? 1 : 0, 1, &stop, &meta))
+		if (prof_begin(x == 0 ? 1 : 0, 1, UINT64_C(1) << nqb, &stop,
+			    &meta))
 			return -1;
 		int rc;
 		switch (g_plain_form) {
@@ -1519,6 +1742,21 @@ int ph2hip_set_fused_form(uint32_t pipe_mask, uint64_t persistent_wgs)
 	return 0;
 }
 
+int ph2hip_set_fused_dma(uint32_t dma_mask, uint64_t wgs)
+{
+	if (ensure_init())
+		return -1;
+	if (!g_pipe_init) {
+		const char *e = getenv("PHASE2_HIP_PIPE");
+		g_pipe_mask = e ? (uint32_t)strtoul(e, nullptr, 0) : 0;
+		g_pipe_init = true;
+	}
+	g_dma_mask = dma_mask;
+	for (int tb = 0; tb <= PH2HIP_TILE_MAX; tb++)
+		g_dma_wgs[tb] = wgs;   // 0: resident workgroups, set at first use
+	return 0;
+}
+
 int ph2hip_mark(int slot)
 {
 	if (ensure_init())
@@ -1675,7 +1913,8 @@ int ph2hip_rotate_batched(void *amp, uint64_t namp_total,
 			ph2hip_launch_rec meta{};
 			meta.tile_bits = tile_bits;
 			meta.ngroups = ni;
-			if (!graph && prof_begin(3, ni, &stop, &meta)) {
+			if (!graph && prof_begin(3, ni, ltiles[p] << tile_bits,
+				    &stop, &meta)) {
 				rc = -1;
 				break;
 			}

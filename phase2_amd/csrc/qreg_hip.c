@@ -40,7 +40,7 @@ struct qreg_hip {
 };
 
 static int g_mode = -1;
-static uint32_t g_tile = 12, g_low = 6;
+static uint32_t g_tile = 12, g_low = 4;
 static ph2_sched_fn g_sched_fn;
 static void *g_sched_ctx;
 

@@ -177,6 +177,22 @@ def trott_n33():
     return dict(hm=hm, det_idx=idx, det_cf=cf, delta=0.02, steps=8)
 
 
+def cmpsit_n30():
+    """Composite integrator, n=30: 60,000 terms (30 Z + 435 ZZ + 59,534
+    four-index, decayed, + identity), 64-determinant random state with 15
+    ones each.  The 2,000 largest |c| form the deterministic part (angle
+    delta = 0.02 on cf), the rest is sampled, 512 draws per half-step with
+    angle lambda_r * delta / 512 (lambda_r = sum |c| of the sampled part),
+    32 steps, one sample."""
+    hm = hamiltonian(30, 0, seed=0, decay=True, n_total=60_000)
+    idx, cf = multidet_random(30, 64, 15, seed=0)
+    length, depth, delta = 2000, 512, 0.02
+    lam_r = float(np.sum(np.abs(hm.cf[length:])))
+    return dict(hm=hm, det_idx=idx, det_cf=cf, length=length, depth=depth,
+                steps=32, samples=1, angle_det=delta,
+                angle_rand=lam_r * delta / depth, seed=0x5EED0030)
+
+
 def hamiltonian_vec(n: int, n_four: int, seed: int, *, sigma_z: float = 0.5,
                     sigma_zz: float = 0.05, sigma_4: float = 0.01,
                     decay: bool = True) -> Hamiltonian:

@@ -10,10 +10,15 @@
    GPU, 10% held out.
 3. Evaluation on 33-qubit blocks never used for training: the same blocks
    are swept with (a) the greedy cut at tile 12 / low 4, (b) the greedy cut
-   at every tile/low setting, keeping the fastest (exhaustive tuning of the
-   fixed setting), and (c) the learned schedule; wall time per block after
-   a synchronisation, and predicted-vs-measured error per pass.
-Writes costmodel_params.npz, records.npy and report.json into DIR.
+   at five tile/low settings around the default, keeping the fastest
+   (exhaustive tuning of the fixed setting), and (c) the learned schedule;
+   wall time per block after a synchronisation, and predicted-vs-measured
+   error per pass.
+4. Evaluation on the composite integrator at 30 qubits (synth.cmpsit_n30):
+   greedy, exhaustive and learned schedules over whole cmpsit steps, with
+   the overlaps compared bit for bit.
+Writes costmodel_params.npz, records.npy, report.json and
+report_cmpsit.json into DIR (--eval-only PARAMS runs step 4 alone).
 """
 
 from __future__ import annotations
@@ -52,7 +57,7 @@ def collect_width(nqb, hm, blocks, rng):
     return rec
 
 
-def time_blocks(reg, xm, zm, phi, blocks):
+def time_blocks(reg, xm, zm, phi, blocks, what=""):
     out = []
     for b in blocks:
         s = slice(b * 1024, (b + 1) * 1024)
@@ -61,7 +66,64 @@ def time_blocks(reg, xm, zm, phi, blocks):
         reg.rotate(xm[s], zm[s], phi[s])
         nat.synchronize()
         out.append(time.perf_counter() - t0)
+    log(f"eval {what}: {sum(out):.3f}s")
     return out
+
+
+def eval_cmpsit(params, steps=1, log=log):
+    """Composite integrator at 30 qubits (synth.cmpsit_n30, `steps` steps):
+    greedy 12/4, greedy at five tile/low settings (exhaustive tuning of the
+    fixed setting), learned; overlaps must agree bit for bit."""
+    w = synth.cmpsit_n30()
+    hm = w["hm"]
+    prep = dict(det_idx=w["det_idx"], det_cf=w["det_cf"])
+
+    def run():
+        nat.synchronize()
+        t0 = time.perf_counter()
+        out, _ = nat.cmpsit(hm, prep, 1, steps, w["length"], w["depth"],
+                            w["angle_det"], w["angle_rand"], w["seed"],
+                            want_state=False)
+        nat.synchronize()
+        return out, time.perf_counter() - t0
+
+    res = {}
+    outs = {}
+    best = (None, 1e300)
+    for tb, low in ((12, 4), (12, 3), (12, 5), (11, 4), (11, 5)):
+        nat.set_default(nat.FUSED, tb, low)
+        outs[(tb, low)], t = run()
+        res[f"greedy_{tb}_{low}_s"] = t
+        log(f"cmpsit greedy {tb}/{low}: {t:.3f}s")
+        if t < best[1]:
+            best = ((tb, low), t)
+    nat.set_default(nat.FUSED, 12, 4)
+    cm.install(params, low=4, min_qb=28)
+    nat.prof_enable(True)
+    nat.prof_records(reset=True)
+    out_l, t_l = run()
+    lrec = nat.prof_records(reset=True)
+    nat.prof_enable(False)
+    hits, misses = cm.stats()
+    cm.uninstall()
+    log(f"cmpsit learned: {t_l:.3f}s")
+    Xl, yl, _ = cm.features_from_records(lrec)
+    m = cm.MLP()
+    m.set_params(params)
+    pl = m.forward(Xl)
+    m.close()
+    err = np.abs(np.exp(pl - yl) - 1.0)
+    same = all(np.array_equal(o, out_l) for o in outs.values())
+    res.update(
+        steps=steps, learned_s=t_l,
+        learned_vs_greedy=res["greedy_12_4_s"] / t_l,
+        exhaustive_best=dict(tile=best[0][0], low=best[0][1], s=best[1]),
+        learned_vs_exhaustive=best[1] / t_l,
+        overlaps_bit_identical=bool(same),
+        pass_median_rel_err=float(np.median(err)) if len(err) else None,
+        pass_p90_rel_err=float(np.quantile(err, 0.9)) if len(err) else None,
+        passes=int(len(err)), cache_hits=hits, cache_misses=misses)
+    return res
 
 
 def main(argv=None):
@@ -72,10 +134,20 @@ def main(argv=None):
     ap.add_argument("--blocks30", type=int, default=36)
     ap.add_argument("--blocks33", type=int, default=10)
     ap.add_argument("--epochs", type=int, default=250)
+    ap.add_argument("--eval-only", default="",
+                    help="parameter file: skip collection and training, "
+                    "evaluate on the composite integrator only")
+    ap.add_argument("--cmpsit-steps", type=int, default=1)
     args = ap.parse_args(argv)
     out = Path(args.out)
     out.mkdir(parents=True, exist_ok=True)
     nat.init()
+    if args.eval_only:
+        params, _ = cm.load(args.eval_only)
+        res = eval_cmpsit(params, args.cmpsit_steps)
+        (out / "report_cmpsit.json").write_text(json.dumps(res, indent=1))
+        log(json.dumps(res))
+        return
     rng = np.random.default_rng(2024)
 
     recs = []
@@ -119,13 +191,13 @@ def main(argv=None):
 
     # evaluation on held-out 33-qubit blocks
     res = {}
-    t_greedy = time_blocks(reg, xm33, zm33, phi33, eval_blocks)
+    t_greedy = time_blocks(reg, xm33, zm33, phi33, eval_blocks, "greedy 12/4")
     res["greedy_12_4_s"] = t_greedy
     best = (None, 1e300)
-    for tb, low in ((12, 3), (12, 4), (12, 5), (12, 6), (11, 3), (11, 4),
-                    (11, 5), (10, 4)):
+    for tb, low in ((12, 3), (12, 4), (12, 5), (11, 4), (11, 5)):
         reg.configure(nat.FUSED, tb, low)
-        t = sum(time_blocks(reg, xm33, zm33, phi33, eval_blocks))
+        t = sum(time_blocks(reg, xm33, zm33, phi33, eval_blocks,
+                            f"greedy {tb}/{low}"))
         res[f"greedy_{tb}_{low}_total_s"] = t
         if t < best[1]:
             best = ((tb, low), t)
@@ -134,10 +206,10 @@ def main(argv=None):
     reg.configure(nat.FUSED, 12, 4)
     cm.install(params, low=4, min_qb=28)
     # first sweep computes and caches the schedules, the second is timed
-    time_blocks(reg, xm33, zm33, phi33, eval_blocks)
+    time_blocks(reg, xm33, zm33, phi33, eval_blocks, "learned, first visit")
     nat.prof_enable(True)
     nat.prof_records(reset=True)
-    t_learned = time_blocks(reg, xm33, zm33, phi33, eval_blocks)
+    t_learned = time_blocks(reg, xm33, zm33, phi33, eval_blocks, "learned")
     lrec = nat.prof_records(reset=True)
     nat.prof_enable(False)
     Xl, yl, lrec = cm.features_from_records(lrec)
@@ -166,6 +238,9 @@ def main(argv=None):
         history=info["history"])
     (out / "report.json").write_text(json.dumps(res, indent=1))
     log(json.dumps({k: v for k, v in res.items() if k != "history"}))
+    res_c = eval_cmpsit(params, args.cmpsit_steps)
+    (out / "report_cmpsit.json").write_text(json.dumps(res_c, indent=1))
+    log(json.dumps(res_c))
 
 
 if __name__ == "__main__":
