@@ -90,13 +90,6 @@ int ph2hip_rotate_fused(void *amp, uint32_t nqb, const struct ph2hip_rot *rot,
  * workgroup per tile.  Initial mask from PHASE2_HIP_PIPE, else 0. */
 int ph2hip_set_fused_form(uint32_t pipe_mask, uint64_t persistent_wgs);
 
-/* Bit TB of dma_mask (TB <= 12) selects the double-buffered persistent form
- * whose tile loads go straight from HBM into LDS (gfx950 global_load_lds),
- * for passes with at least 4 contiguous low bits; it takes precedence over
- * the pipelined form.  wgs: its workgroup count (0: as many as are resident
- * on the device at once).  Initial mask from PHASE2_HIP_DMA, else 0.  The
- * state is bit-identical in every form. */
-int ph2hip_set_fused_dma(uint32_t dma_mask, uint64_t wgs);
 
 /* Shape of the fused pass over rot[0..n) without launching it: shape[0] =
  * contiguous low bits L, shape[1] = gathered dimensions g, shape[2] =

@@ -361,15 +361,6 @@ def set_fused_form(pipe_mask: int, persistent_wgs: int = 0):
         _err("ph2hip_set_fused_form")
 
 
-def set_fused_dma(dma_mask: int, wgs: int = 0):
-    """Bit TB of dma_mask: double-buffered LDS-DMA fused kernel for tile TB
-    (wgs workgroups; 0 = all resident ones)."""
-    L = lib()
-    L.ph2hip_set_fused_dma.restype = C.c_int
-    if L.ph2hip_set_fused_dma(C.c_uint32(dma_mask), C.c_uint64(wgs)) != 0:
-        _err("ph2hip_set_fused_dma")
-
-
 def set_plain_form(form: int):
     """2, 4 or 8 pairs per thread; +16 for non-temporal stores."""
     lib().ph2hip_set_plain_form(C.c_int(form))

@@ -380,28 +380,7 @@ __device__ __forceinline__ uint64_t tile_base(const FusedPass &P, uint64_t t)
 // registers while the current tile is rotated in LDS, and its writes are
 // issued without waiting.
 // All micro-groups of a pass on one tile held in LDS.
-// Workgroup barrier.  LDSONLY: orders LDS accesses only (waits for the
-// wave's LDS operations, not for its outstanding global loads and stores),
-// so HBM traffic issued before it stays in flight.
-template <bool LDSONLY>
-__device__ __forceinline__ void tile_barrier()
-{
-	if constexpr (LDSONLY) {
-		// lgkmcnt(0) (this wave's LDS accesses are done; vmcnt and expcnt
-		// at their maxima, i.e. not waited for), then the barrier.  A
-		// workgroup fence would also wait for the LDS-writing HBM loads.
-		// The empty asm statements keep the compiler from moving any
-		// memory access across.
-		__asm__ volatile("" ::: "memory");
-		__builtin_amdgcn_s_waitcnt(0xC07F);
-		__builtin_amdgcn_s_barrier();
-		__asm__ volatile("" ::: "memory");
-	} else {
-		__syncthreads();
-	}
-}
-
-template <int TB, int D, bool LDSONLY = false>
+template <int TB, int D>
 __device__ __forceinline__ void tile_groups(v2d *tile, const FusedPass &P,
 	uint64_t base, uint32_t tid)
 {
@@ -416,7 +395,7 @@ __device__ __forceinline__ void tile_groups(v2d *tile, const FusedPass &P,
 		const uint32_t gu[4] = { uw0 & 0xffffu, uw0 >> 16, uw1 & 0xffffu,
 			uw1 >> 16 };
 		const uint32_t jw[3] = { jw0, jw1, jw2 };
-		tile_barrier<LDSONLY>();
+		__syncthreads();
 		for (uint32_t j = tid; j < NSUB; j += kBlock) {
 			// coset base: j's bits spread over the free tile bits
 			uint32_t bu = 0;
@@ -487,7 +466,60 @@ __device__ __forceinline__ void tile_groups(v2d *tile, const FusedPass &P,
 			}
 		}
 	}
-	tile_barrier<LDSONLY>();
+	__syncthreads();
+}
+
+// Direct HBM->LDS tile loads (gfx950 global_load_lds_dwordx4): the data
+// does not pass through registers, so all of a thread's loads of the tile
+// are in flight at once whatever the register pressure of the rotation
+// code.  Lane l of wave w in load step m fills LDS slot s = m*256 + 64w + l,
+// i.e. tile coordinate lds_slot(s) = s ^ f with f = ((s >> 4) ^ (s >> 8)) &
+// 15 (the swizzle is an involution).  f only touches coordinate bits 0..3,
+// which lie inside a row when L >= 4, so the lane reads column (s ^ f) &
+// lmask of the row of s: every 16 lanes still cover one 256-byte run.
+// Requires L >= 4.
+#define PH2_LDS(p) ((__attribute__((address_space(3))) void *)(p))
+
+// s_waitcnt with vmcnt <= N and no wait on the other counters (gfx9
+// encoding: vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8).
+template <uint32_t N>
+__device__ __forceinline__ void vm_wait()
+{
+	static_assert(N < 64, "vmcnt field");
+	__asm__ volatile("" ::: "memory");
+	__builtin_amdgcn_s_waitcnt((N & 15u) | ((N >> 4) << 14) | (7u << 4) |
+		(15u << 8));
+	__asm__ volatile("" ::: "memory");
+}
+
+template <int TB>
+__device__ __forceinline__ void dma_tile(const v2d *__restrict__ a,
+	v2d *__restrict__ buf,
+	const FusedPass &P, uint64_t bo, uint32_t tid)
+{
+	constexpr uint32_t PER = (1u << TB) / kBlock;
+	const uint32_t lmask = (1u << pass_L(P)) - 1;
+	const uint32_t wave0 = __builtin_amdgcn_readfirstlane(tid & ~63u);
+#pragma unroll
+	for (uint32_t m = 0; m < PER; m++) {
+		const uint32_t u = tid + m * kBlock;
+		const uint32_t f = ((tid >> 4) ^ m) & 15u;
+		__builtin_amdgcn_global_load_lds(
+			(const void *)(a + (bo ^ P.roff[m] ^ ((u ^ f) & lmask))),
+			PH2_LDS(buf + m * kBlock + wave0), 16, 0, 0);
+	}
+}
+
+template <int TB>
+__device__ __forceinline__ void store_tile(v2d *__restrict__ a,
+	const v2d *buf, const FusedPass &P, uint64_t bo, uint32_t tid)
+{
+	constexpr uint32_t PER = (1u << TB) / kBlock;
+	const uint32_t lmask = (1u << pass_L(P)) - 1;
+#pragma unroll
+	for (uint32_t m = 0; m < PER; m++)
+		a[bo ^ P.roff[m] ^ ((tid + m * kBlock) & lmask)] =
+			buf[lds_slot(tid + m * kBlock)];
 }
 
 // Load one tile from HBM into LDS, apply the pass, write it back.
@@ -504,15 +536,18 @@ __device__ __forceinline__ void fused_tile(v2d *__restrict__ a,
 			wt ^= P.w[b];
 	const uint64_t base = tile_base(P, t);
 	const uint64_t bo = base ^ wt;
+	if (L >= 4) {
+		dma_tile<TB>(a, tile, P, bo, tid);
+		vm_wait<0>();
+	} else {
+		// staged through registers, a few loads in flight at a time
 #pragma unroll 4
-	for (uint32_t m = 0; m < PER; m++)
-		tile[lds_slot(tid + m * kBlock)] =
-			a[bo ^ P.roff[m] ^ ((tid + m * kBlock) & lmask)];
+		for (uint32_t m = 0; m < PER; m++)
+			tile[lds_slot(tid + m * kBlock)] =
+				a[bo ^ P.roff[m] ^ ((tid + m * kBlock) & lmask)];
+	}
 	tile_groups<TB, D>(tile, P, base, tid);
-#pragma unroll
-	for (uint32_t m = 0; m < PER; m++)
-		a[bo ^ P.roff[m] ^ ((tid + m * kBlock) & lmask)] =
-			tile[lds_slot(tid + m * kBlock)];
+	store_tile<TB>(a, tile, P, bo, tid);
 }
 
 // One tile per workgroup (grid = number of tiles); HBM latency is hidden
@@ -605,116 +640,6 @@ __global__ __launch_bounds__(kBlock) void k_rot_fused_pipe(
 		for (uint32_t m = 0; m < PER; m++)
 			a[bo ^ P.roff[m] ^ ((tid + m * kBlock) & lmask)] =
 				tile[lds_slot(tid + m * kBlock)];
-	}
-}
-
-// Double-buffered persistent pass with direct HBM->LDS loads (gfx950
-// global_load_lds_dwordx4): while a workgroup rotates tile t in one LDS
-// buffer, the loads of its next tile land in the other without passing
-// through registers, and the write-back of t drains behind them.  Lane l of
-// wave w in load step m fills LDS slot s = m*256 + 64w + l, i.e. tile
-// coordinate lds_slot(s) = s ^ f with f = ((s >> 4) ^ (s >> 8)) & 15 (the
-// swizzle is an involution).  f only touches coordinate bits 0..3, which
-// lie inside a row when L >= 4, so the lane reads column (s ^ f) & lmask of
-// the row of s: every 16 lanes still cover one 256-byte run.  Requires
-// L >= 4 (checked on the host).
-#define PH2_LDS(p) ((__attribute__((address_space(3))) void *)(p))
-
-// s_waitcnt with vmcnt <= N and no wait on the other counters (gfx9
-// encoding: vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8).
-template <uint32_t N>
-__device__ __forceinline__ void vm_wait()
-{
-	static_assert(N < 64, "vmcnt field");
-	__asm__ volatile("" ::: "memory");
-	__builtin_amdgcn_s_waitcnt((N & 15u) | ((N >> 4) << 14) | (7u << 4) |
-		(15u << 8));
-	__asm__ volatile("" ::: "memory");
-}
-
-template <int TB>
-__device__ __forceinline__ void dma_tile(const v2d *__restrict__ a,
-	v2d *__restrict__ buf,
-	const FusedPass &P, uint64_t bo, uint32_t tid)
-{
-	constexpr uint32_t PER = (1u << TB) / kBlock;
-	const uint32_t lmask = (1u << pass_L(P)) - 1;
-	const uint32_t wave0 = __builtin_amdgcn_readfirstlane(tid & ~63u);
-#pragma unroll
-	for (uint32_t m = 0; m < PER; m++) {
-		const uint32_t u = tid + m * kBlock;
-		const uint32_t f = ((tid >> 4) ^ m) & 15u;
-		__builtin_amdgcn_global_load_lds(
-			(const void *)(a + (bo ^ P.roff[m] ^ ((u ^ f) & lmask))),
-			PH2_LDS(buf + m * kBlock + wave0), 16, 0, 0);
-	}
-}
-
-template <int TB>
-__device__ __forceinline__ void store_tile(v2d *__restrict__ a,
-	const v2d *buf, const FusedPass &P, uint64_t bo, uint32_t tid)
-{
-	constexpr uint32_t PER = (1u << TB) / kBlock;
-	const uint32_t lmask = (1u << pass_L(P)) - 1;
-#pragma unroll
-	for (uint32_t m = 0; m < PER; m++)
-		a[bo ^ P.roff[m] ^ ((tid + m * kBlock) & lmask)] =
-			buf[lds_slot(tid + m * kBlock)];
-}
-
-// One half of the double-buffered loop: barrier (tile t has landed in
-// `cur` in every wave; nobody reads `nxt` any more), prefetch tile tn into
-// `nxt`, rotate t in `cur`, write it back.  cur and nxt are restrict
-// parameters of one inlined call, so the compiler can tell that accesses to
-// cur do not alias the loads landing in nxt and adds no wait for them.
-template <int TB, int D>
-__device__ __forceinline__ void dma_half(v2d *__restrict__ a,
-	v2d *__restrict__ cur, v2d *__restrict__ nxt, const FusedPass &P,
-	uint64_t t, uint64_t tn, uint64_t ntiles, uint64_t wt, uint32_t tid)
-{
-	tile_barrier<true>();
-	if (tn < ntiles)
-		dma_tile<TB>(a, nxt, P, tile_base(P, tn) ^ wt, tid);
-	const uint64_t b0 = tile_base(P, t);
-	tile_groups<TB, D, true>(cur, P, b0, tid);
-	store_tile<TB>(a, cur, P, b0 ^ wt, tid);
-}
-
-template <int TB, int D>
-__global__ __launch_bounds__(kBlock) void k_rot_fused_dma(
-	double2 *__restrict__ amp, const FusedPass P, uint64_t ntiles)
-{
-	constexpr uint32_t PER = (1u << TB) / kBlock;
-	__shared__ v2d bufA[1u << TB];
-	__shared__ v2d bufB[1u << TB];
-	v2d *__restrict__ a = reinterpret_cast<v2d *>(amp);
-	const uint32_t L = pass_L(P), tid = threadIdx.x;
-	const uint32_t g = pass_g(P);
-	uint64_t wt = 0;   // gathered offset of this thread's row part
-	for (uint32_t b = 0; b < g; b++)
-		if ((tid >> L) >> b & 1)
-			wt ^= P.w[b];
-	const uint64_t G = gridDim.x;
-	uint64_t t = blockIdx.x;
-	if (t >= ntiles)
-		return;           // whole workgroup
-	dma_tile<TB>(a, bufA, P, tile_base(P, t) ^ wt, tid);
-	vm_wait<0>();
-	// After each half only that half's PER stores may still be in flight
-	// once the prefetch has landed (memory operations complete in issue
-	// order): wait for vmcnt <= PER.
-	for (;;) {
-		const uint64_t t1 = t + G;
-		dma_half<TB, D>(a, bufA, bufB, P, t, t1, ntiles, wt, tid);
-		if (t1 >= ntiles)
-			break;
-		vm_wait<PER>();
-		const uint64_t t2 = t1 + G;
-		dma_half<TB, D>(a, bufB, bufA, P, t1, t2, ntiles, wt, tid);
-		if (t2 >= ntiles)
-			break;
-		vm_wait<PER>();
-		t = t2;
 	}
 }
 
@@ -1005,8 +930,6 @@ constexpr int group_depth(int tb)
 
 bool g_pipe_init = false;
 uint32_t g_pipe_mask = 0;   // bit TB set: persistent pipelined form
-uint32_t g_dma_mask = 0;    // bit TB set: double-buffered LDS-DMA form
-uint64_t g_dma_wgs[PH2HIP_TILE_MAX + 1];   // its grid per tile size
 
 template <int TB>
 int launch_fused(double2 *a, uint32_t nqb, const FusedPass &P,
@@ -1015,9 +938,6 @@ int launch_fused(double2 *a, uint32_t nqb, const FusedPass &P,
 	if (!g_pipe_init) {
 		const char *e = getenv("PHASE2_HIP_PIPE");
 		g_pipe_mask = e ? (uint32_t)strtoul(e, nullptr, 0) : 0;
-		const char *d = getenv("PHASE2_HIP_DMA");
-		if (d)
-			g_dma_mask = (uint32_t)strtoul(d, nullptr, 0);
 		g_pipe_init = true;
 	}
 	const uint64_t ntiles = UINT64_C(1) << (nqb - TB);
@@ -1034,25 +954,6 @@ int launch_fused(double2 *a, uint32_t nqb, const FusedPass &P,
 	meta.zunion = zunion;
 	if (prof_begin(2, P.nterms, UINT64_C(1) << nqb, &stop, &meta))
 		return -1;
-	if constexpr (TB <= 12) {
-		if ((g_dma_mask >> TB & 1) && P.L >= 4) {
-			if (!g_dma_wgs[TB]) {
-				int per_cu = 0;
-				HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-					&per_cu, k_rot_fused_dma<TB, group_depth(TB)>,
-					kBlock, 0));
-				g_dma_wgs[TB] = (uint64_t)(per_cu > 0 ? per_cu : 1) *
-					(uint64_t)g_ncu;
-			}
-			const uint64_t grid = ntiles < g_dma_wgs[TB] ? ntiles
-				: g_dma_wgs[TB];
-			hipLaunchKernelGGL((k_rot_fused_dma<TB, group_depth(TB)>),
-				dim3((uint32_t)grid), dim3(kBlock), 0, g_stream, a, P,
-				ntiles);
-			HIPCHK(hipGetLastError());
-			return prof_end(stop);
-		}
-	}
 	if (g_pipe_mask >> TB & 1) {
 		const uint64_t grid = ntiles < g_persist_wgs ? ntiles
 			: g_persist_wgs;
@@ -1739,21 +1640,6 @@ int ph2hip_set_fused_form(uint32_t pipe_mask, uint64_t persistent_wgs)
 	g_pipe_init = true;
 	if (persistent_wgs)
 		g_persist_wgs = persistent_wgs;
-	return 0;
-}
-
-int ph2hip_set_fused_dma(uint32_t dma_mask, uint64_t wgs)
-{
-	if (ensure_init())
-		return -1;
-	if (!g_pipe_init) {
-		const char *e = getenv("PHASE2_HIP_PIPE");
-		g_pipe_mask = e ? (uint32_t)strtoul(e, nullptr, 0) : 0;
-		g_pipe_init = true;
-	}
-	g_dma_mask = dma_mask;
-	for (int tb = 0; tb <= PH2HIP_TILE_MAX; tb++)
-		g_dma_wgs[tb] = wgs;   // 0: resident workgroups, set at first use
 	return 0;
 }
 

@@ -299,13 +299,12 @@ def test_fused_pipelined_form_bit_identical(oracle, pipe_tile):
         nat.set_plain_form(2)
 
 
-@pytest.mark.parametrize("tile,low,wgs", [(10, 4, 16), (11, 4, 8),
-                                          (11, 5, 0), (12, 4, 8), (12, 3, 8)])
-def test_fused_dma_form_bit_identical(oracle, tile, low, wgs):
-    """The double-buffered LDS-DMA fused kernel (passes with L >= 4; the
-    others fall back to the one-tile-per-workgroup form) leaves the state
-    bit-identical to the oracle.  With few workgroups every workgroup runs
-    both buffer halves several times."""
+@pytest.mark.parametrize("tile,low", [(10, 4), (11, 5), (12, 4), (12, 3)])
+def test_fused_direct_load_bit_identical(oracle, tile, low):
+    """Tiles with at least 4 contiguous low bits are loaded straight into LDS
+    (lanes permuted within 256-byte runs to match the LDS swizzle), the
+    others through registers; both leave the state bit-identical to the
+    oracle."""
     n = 18
     rng = np.random.default_rng(60 + tile + low)
     st = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
@@ -314,12 +313,8 @@ def test_fused_dma_form_bit_identical(oracle, tile, low, wgs):
     xm, zm = hm.xm[perm], hm.zm[perm]
     phi = 0.3 * hm.cf[perm] / np.max(np.abs(hm.cf))
     want = oracle.rotate(st, xm, zm, phi)
-    try:
-        nat.set_fused_dma(1 << tile, wgs)
-        got = nat.rotate(st, xm, zm, phi, nat.FUSED, tile, low)
-        assert np.array_equal(got, want)
-    finally:
-        nat.set_fused_dma(0)
+    got = nat.rotate(st, xm, zm, phi, nat.FUSED, tile, low)
+    assert np.array_equal(got, want)
 
 
 def _batch_regs(widths, depth=120, samples=2):
