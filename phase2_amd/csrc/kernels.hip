@@ -328,10 +328,16 @@ __device__ __forceinline__ void coset_term(double (&re)[1 << D],
 	double (&im)[1 << D], uint32_t ztp, double sb, double c)
 {
 	constexpr int H = 31 - __builtin_clz(XT);
+	// sb's words, kept opaque so that the compiler materialises them once
+	// per term instead of re-deriving the per-thread sign in every pair;
+	// per pair only the uniform sign bit (scalar unit) is XORed in.
+	int sbh = __double2hiint(sb), sbl = __double2loint(sb);
+	__asm__("" : "+v"(sbh));
 #pragma unroll
 	for (int q = 0; q < (1 << (D - 1)); q++) {
 		const int w = ((q >> H) << (H + 1)) | (q & ((1 << H) - 1));
-		const double si = flip_sign(sb, __popc(w & ztp) & 1u);
+		const double si = __hiloint2double(
+			sbh ^ (int)((__popc(w & ztp) & 1u) << 31), sbl);
 		double &xr = re[w], &xi = im[w];
 		double &yr = re[w ^ XT], &yi = im[w ^ XT];
 		double nxr, nxi, nyr, nyi;
