@@ -223,6 +223,12 @@ def plain_rate(nat, nqb, xm, zm, phi):
                 diag_launches=pp["plain_diag"]["launches"])
 
 
+def fused_name(tile, batched=False):
+    """Kernel name as rocprofv3 prints it (template arguments TB, D)."""
+    d = 4 if tile >= 12 else tile - 8
+    return f"k_rot_fused{'_batch' if batched else ''}<{tile}, {d}>"
+
+
 def spread(n, k):
     return np.unique(np.linspace(0, n - 1, min(n, k)).astype(int))
 
@@ -301,7 +307,7 @@ def bench_trott_n33(args, nat, costmodel, schedule):
         plain = plain_rate(nat, nqb, xm[sel], zm[sel], phi[sel])
     return dict(
         value=value, elapsed=elapsed, norm2=norm2, plain=plain,
-        roofline=roofline(f"k_rot_fused<{args.tile}>", fz),
+        roofline=roofline(fused_name(args.tile), fz),
         config={"workload": f"trott1_n{nqb}_lex_sweep_blocks", "nqb": nqb,
                 "terms": nterms, "terms_per_step": 1024,
                 "blocks_timed": blocks, "blocks_in_sweep": nblocks,
@@ -345,8 +351,8 @@ def _timed_driver(nat, run, steps, warmup):
 
 
 def _dominant(prof, tile):
-    kinds = {"fused": f"k_rot_fused<{tile}>", "plain_pair": "k_rot_pair",
-             "plain_diag": "k_rot_diag", "batched": f"k_rot_fused_batch<{tile}>"}
+    kinds = {"fused": fused_name(tile), "plain_pair": "k_rot_pair",
+             "plain_diag": "k_rot_diag", "batched": fused_name(tile, True)}
     k = max(kinds, key=lambda n: prof[n]["ms"])
     return kinds[k], prof[k]
 
@@ -495,7 +501,7 @@ def bench_qdrift_batch(args, nat, costmodel, schedule):
         plain={"amp_updates_per_s": sp["amp_updates"] / sp["seconds"],
                "subset": "first 64 registers, 1 sample, one launch per "
                "rotation", "launches": sp["launches"]},
-        roofline=roofline(f"k_rot_fused_batch<{st['tile_bits']}>",
+        roofline=roofline(fused_name(st["tile_bits"], True),
                           prof["batched"]),
         config={"workload": "qdrift_batch_512", "registers": len(regs),
                 "widths": [int(min(widths)), int(max(widths))],
@@ -585,7 +591,7 @@ def main(argv=None):
         schedule = "learned" if costmodel.PARAMS_PATH.is_file() else "greedy"
     if schedule == "learned":
         params, _ = costmodel.load()
-        costmodel.install(params, low=args.low, min_qb=28)
+        costmodel.install(params, min_qb=28)
 
     r = BENCH[args.workload](args, nat, costmodel, schedule)
     if schedule == "learned":

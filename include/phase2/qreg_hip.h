@@ -38,16 +38,21 @@ void qreg_hip_configure(struct qreg *reg, int mode, uint32_t tile_bits,
 	uint32_t low_bits);
 
 /* Fusion schedule of one block of terms: npass passes covering the terms
- * in order, pass p taking len[p] terms in tiles of tile_bits[p] bits. */
+ * in order, pass p taking len[p] terms in tiles of tile_bits[p] bits with
+ * at least low_bits[p] contiguous low bits. */
 struct ph2_sched {
 	size_t npass;
-	uint32_t *len;
-	uint32_t *tile_bits;
-	uint32_t low_bits;
+	uint32_t *len;        /* terms per pass */
+	uint32_t *tile_bits;  /* tile width per pass */
+	uint32_t *low_bits;   /* requested contiguous low bits per pass */
 	size_t cap;
 };
 
 void ph2_sched_free(struct ph2_sched *s);
+/* Room for cap passes; append one pass. 0 or -1 (out of memory). */
+int ph2_sched_reserve(struct ph2_sched *s, size_t cap);
+int ph2_sched_push(struct ph2_sched *s, uint32_t len, uint32_t tile_bits,
+	uint32_t low_bits);
 
 /* Greedy cut: extend each pass while its flip masks span at most
  * tile_bits - low_bits dimensions beyond the low bits and it holds at most

@@ -90,11 +90,12 @@ int ph2_reg_prep_multidet(void *reg, size_t ndet, const uint64_t *didx,
 	const double *dcf);
 int ph2_reg_rotate(void *reg, size_t n, const uint64_t *xm,
 	const uint64_t *zm, const double *phi);
-/* Rotations with an explicit fused schedule (pass lengths and per-pass tile
- * bits), bypassing the scheduler; for timing records and tests. */
+/* Rotations with an explicit fused schedule (pass lengths, per-pass tile
+ * bits and contiguous low bits), bypassing the scheduler; for timing
+ * records and tests. */
 int ph2_reg_rotate_sched(void *reg, size_t n, const uint64_t *xm,
 	const uint64_t *zm, const double *phi, const uint32_t *pass_len,
-	const uint32_t *tile_bits, size_t npass, uint32_t low_bits);
+	const uint32_t *tile_bits, const uint32_t *low_bits, size_t npass);
 int ph2_reg_overlap_multidet(void *reg, size_t ndet, const uint64_t *didx,
 	const double *dcf, double *out);
 int ph2_reg_norm2(void *reg, double *out);

@@ -72,7 +72,7 @@ int ph2hip_set_plain_form(int form);
 /* Fused sweep: the terms are applied in order, pass by pass; pass p covers
  * the next pass_len[p] terms and runs as one HBM pass over LDS tiles of
  * 2^tile_bits[p] amplitudes, each tile a coset of a subspace that contains
- * the low `low_bits` index bits and every flip mask of the pass.  The state
+ * the low low_bits[p] index bits and every flip mask of the pass.  The state
  * is bit-identical to ph2hip_rotate_plain.  Fails (-1, nothing applied) if a
  * pass does not fit its tile (span too large), tile_bits is outside
  * [PH2HIP_TILE_MIN, PH2HIP_TILE_MAX] or above nqb, or a pass holds more than
@@ -82,7 +82,7 @@ int ph2hip_set_plain_form(int form);
 #define PH2HIP_PASS_MAX 48
 int ph2hip_rotate_fused(void *amp, uint32_t nqb, const struct ph2hip_rot *rot,
 	size_t n, const uint32_t *pass_len, const uint32_t *tile_bits,
-	size_t npass, uint32_t low_bits);
+	size_t npass, const uint32_t *low_bits);
 
 /* Kernel form of the fused pass per tile size: bit TB of pipe_mask selects
  * the persistent, software-pipelined form (persistent_wgs workgroups; 0

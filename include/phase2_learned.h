@@ -13,8 +13,10 @@
  * as y = ln(ms) - (nqb - 30) ln 2 (time per 2^30 amplitudes).  For a block of
  * terms the scheduler enumerates passes [i, j) that end where the next term
  * would widen the tile (or at the longest feasible end), for tile widths 11
- * and 12, predicts them all in one batch on the GPU and picks the cut with
- * the smallest predicted total by dynamic programming.  Schedules are cached
+ * and 12 and for low_bits, low_bits + 1 and low_bits + 2 contiguous low
+ * bits, predicts them all in one batch on the GPU and picks the cut, tile
+ * and low bits of every pass with the smallest predicted total by dynamic
+ * programming.  Schedules are cached
  * per block (flip and phase masks, width), so repeated Trotter sweeps pay
  * for scheduling once.  The order of terms never changes, so the state stays
  * bit-identical to the plain sweep.
@@ -40,8 +42,8 @@ void ph2_learned_features(const struct ph2hip_rot *rot, size_t n,
 
 /* Load parameters (packed as ph2hip_mlp) of a PH2_LM_NIN -> nhid -> nhid
  * -> 1 network, and install the learned schedule for fused registers of at
- * least min_qb qubits (narrower ones keep the greedy cut).  low_bits is
- * the requested contiguous low part of every tile. */
+ * least min_qb qubits (narrower ones keep the greedy cut).  low_bits is the
+ * smallest contiguous low part a pass may request. */
 int ph2_learned_install(const float *params, uint32_t nhid, uint32_t low_bits,
 	uint32_t min_qb);
 void ph2_learned_uninstall(void);

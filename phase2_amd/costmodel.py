@@ -183,12 +183,14 @@ def _span_growth(xs, low):
     return out
 
 
-def random_schedule(xm, rng, tiles=(11, 12), low=4, maxlen=48):
-    """Cut a block into passes of random feasible length and tile width."""
-    lens, tbs = [], []
+def random_schedule(xm, rng, tiles=(11, 12), lows=(3, 4, 5), maxlen=48):
+    """Cut a block into passes of random feasible length, tile width and
+    contiguous low bits."""
+    lens, tbs, lbs = [], [], []
     i, n = 0, len(xm)
     while i < n:
         tb = int(rng.choice(tiles))
+        low = int(rng.choice(lows))
         ranks = _span_growth(xm[i:i + maxlen], low)
         jmax = 1
         while jmax < len(ranks) and ranks[jmax] <= tb - low:
@@ -196,11 +198,13 @@ def random_schedule(xm, rng, tiles=(11, 12), low=4, maxlen=48):
         ln = int(rng.integers(1, jmax + 1)) if rng.random() < 0.7 else jmax
         lens.append(ln)
         tbs.append(tb)
+        lbs.append(low)
         i += ln
-    return np.array(lens, np.uint32), np.array(tbs, np.uint32)
+    return (np.array(lens, np.uint32), np.array(tbs, np.uint32),
+            np.array(lbs, np.uint32))
 
 
-def collect(reg, xm, zm, phi, blocks, rng, low=4, log=print):
+def collect(reg, xm, zm, phi, blocks, rng, log=print):
     """Run random schedules over the given 1024-term blocks on a device
     register and return the per-pass timing records."""
     L = nat.lib()
@@ -212,15 +216,15 @@ def collect(reg, xm, zm, phi, blocks, rng, low=4, log=print):
         s = slice(b * 1024, (b + 1) * 1024)
         bx, bz, bp = (np.ascontiguousarray(xm[s]), np.ascontiguousarray(zm[s]),
                       np.ascontiguousarray(phi[s]))
-        lens, tbs = random_schedule(bx, rng, low=low)
+        lens, tbs, lbs = random_schedule(bx, rng)
         rc = L.ph2_reg_rotate_sched(
             C.c_void_p(reg.h), C.c_size_t(len(bx)),
             bx.ctypes.data_as(C.POINTER(C.c_uint64)),
             bz.ctypes.data_as(C.POINTER(C.c_uint64)),
             bp.ctypes.data_as(C.POINTER(C.c_double)),
             lens.ctypes.data_as(C.POINTER(C.c_uint32)),
-            tbs.ctypes.data_as(C.POINTER(C.c_uint32)), C.c_size_t(len(lens)),
-            C.c_uint32(low))
+            tbs.ctypes.data_as(C.POINTER(C.c_uint32)),
+            lbs.ctypes.data_as(C.POINTER(C.c_uint32)), C.c_size_t(len(lens)))
         if rc != 0:
             raise RuntimeError(f"ph2_reg_rotate_sched: {L.ph2_strerror()}")
         if (bi + 1) % 8 == 0:
@@ -271,8 +275,9 @@ def load(path=PARAMS_PATH):
     return d["params"], {k: d[k] for k in d.files if k != "params"}
 
 
-def install(params, low=4, min_qb=28):
-    """Use the learned schedule for fused registers of >= min_qb qubits."""
+def install(params, low=3, min_qb=28):
+    """Use the learned schedule for fused registers of >= min_qb qubits
+    (passes with low .. low + 2 contiguous low bits)."""
     p = np.ascontiguousarray(params, np.float32)
     L = nat.lib()
     L.ph2_learned_install.restype = C.c_int
@@ -283,6 +288,66 @@ def install(params, low=4, min_qb=28):
 
 def uninstall():
     nat.lib().ph2_learned_uninstall()
+
+
+class Sched(C.Structure):
+    """struct ph2_sched (include/phase2/qreg_hip.h)."""
+    _fields_ = [("npass", C.c_size_t), ("len", C.POINTER(C.c_uint32)),
+                ("tile_bits", C.POINTER(C.c_uint32)),
+                ("low_bits", C.POINTER(C.c_uint32)), ("cap", C.c_size_t)]
+
+
+def _rots(xm, zm, phi=None):
+    n = len(xm)
+    rot = (nat.Rot * max(n, 1))()
+    for k in range(n):
+        p = 0.0 if phi is None else float(phi[k])
+        rot[k] = nat.Rot(int(xm[k]), int(zm[k]), math.cos(p), math.sin(p))
+    return rot
+
+
+def _sched_arrays(s):
+    n = s.npass
+    return (np.array(s.len[:n], np.uint32), np.array(s.tile_bits[:n], np.uint32),
+            np.array(s.low_bits[:n], np.uint32))
+
+
+def schedule(xm, zm, nqb, greedy=None):
+    """Schedule of one block: the installed learned one, or greedy=(tile,
+    low).  Returns (pass lengths, tile bits, low bits)."""
+    L = nat.lib()
+    rot = _rots(xm, zm)
+    s = Sched()
+    if greedy is None:
+        L.ph2_learned_schedule.restype = C.c_int
+        rc = L.ph2_learned_schedule(rot, C.c_size_t(len(xm)),
+                                    C.c_uint32(nqb), C.byref(s))
+    else:
+        L.ph2_sched_greedy.restype = C.c_int
+        rc = L.ph2_sched_greedy(rot, C.c_size_t(len(xm)), C.c_uint32(nqb),
+                                C.c_uint32(greedy[0]), C.c_uint32(greedy[1]),
+                                C.byref(s))
+    if rc != 0:
+        raise RuntimeError(f"schedule: {L.ph2_strerror()}")
+    out = _sched_arrays(s)
+    L.ph2_sched_free(C.byref(s))
+    return out
+
+
+def predict(xm, zm, nqb, sched):
+    """Predicted milliseconds per pass of a schedule (installed network)."""
+    L = nat.lib()
+    lens, tbs, lbs = (np.ascontiguousarray(a, np.uint32) for a in sched)
+    s = Sched(len(lens), lens.ctypes.data_as(C.POINTER(C.c_uint32)),
+              tbs.ctypes.data_as(C.POINTER(C.c_uint32)),
+              lbs.ctypes.data_as(C.POINTER(C.c_uint32)), len(lens))
+    ms = np.zeros(len(lens))
+    L.ph2_learned_predict.restype = C.c_int
+    if L.ph2_learned_predict(_rots(xm, zm), C.c_size_t(len(xm)),
+                             C.c_uint32(nqb), C.byref(s),
+                             ms.ctypes.data_as(C.POINTER(C.c_double))) != 0:
+        raise RuntimeError(f"ph2_learned_predict: {L.ph2_strerror()}")
+    return ms
 
 
 def stats():

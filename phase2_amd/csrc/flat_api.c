@@ -357,7 +357,7 @@ int ph2_reg_rotate(void *r, size_t n, const uint64_t *xm, const uint64_t *zm,
 
 int ph2_reg_rotate_sched(void *r, size_t n, const uint64_t *xm,
 	const uint64_t *zm, const double *phi, const uint32_t *pass_len,
-	const uint32_t *tile_bits, size_t npass, uint32_t low_bits)
+	const uint32_t *tile_bits, const uint32_t *low_bits, size_t npass)
 {
 	struct qreg *reg = r;
 	struct ph2hip_rot *rot = malloc(sizeof *rot * (n ? n : 1));

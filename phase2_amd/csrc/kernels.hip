@@ -1382,7 +1382,7 @@ uint32_t ph2hip_span_rank(const struct ph2hip_rot *rot, size_t n,
 
 int ph2hip_rotate_fused(void *amp, uint32_t nqb, const struct ph2hip_rot *rot,
 	size_t n, const uint32_t *pass_len, const uint32_t *tile_bits,
-	size_t npass, uint32_t low_bits)
+	size_t npass, const uint32_t *low_bits)
 {
 	if (ensure_init())
 		return -1;
@@ -1414,7 +1414,7 @@ int ph2hip_rotate_fused(void *amp, uint32_t nqb, const struct ph2hip_rot *rot,
 	size_t k0 = 0;
 	for (size_t p = 0; p < npass; p++) {
 		if (build_pass(passes[p], rot + k0, pass_len[p], tile_bits[p],
-			    low_bits, k0)) {
+			    low_bits[p], k0)) {
 			free(passes);
 			return -1;
 		}

@@ -12,26 +12,40 @@ void ph2_sched_free(struct ph2_sched *s)
 {
 	free(s->len);
 	free(s->tile_bits);
-	s->len = s->tile_bits = NULL;
+	free(s->low_bits);
+	s->len = s->tile_bits = s->low_bits = NULL;
 	s->npass = s->cap = 0;
 }
 
-static int sched_push(struct ph2_sched *s, uint32_t len, uint32_t tb)
+int ph2_sched_reserve(struct ph2_sched *s, size_t cap)
 {
-	if (s->npass == s->cap) {
-		const size_t cap = s->cap ? 2 * s->cap : 64;
-		uint32_t *a = realloc(s->len, sizeof *a * cap);
-		if (!a)
-			return -1;
-		s->len = a;
-		uint32_t *b = realloc(s->tile_bits, sizeof *b * cap);
-		if (!b)
-			return -1;
-		s->tile_bits = b;
-		s->cap = cap;
-	}
+	if (cap <= s->cap)
+		return 0;
+	uint32_t *a = realloc(s->len, sizeof *a * cap);
+	if (!a)
+		return -1;
+	s->len = a;
+	uint32_t *b = realloc(s->tile_bits, sizeof *b * cap);
+	if (!b)
+		return -1;
+	s->tile_bits = b;
+	uint32_t *c = realloc(s->low_bits, sizeof *c * cap);
+	if (!c)
+		return -1;
+	s->low_bits = c;
+	s->cap = cap;
+	return 0;
+}
+
+int ph2_sched_push(struct ph2_sched *s, uint32_t len, uint32_t tb,
+	uint32_t low)
+{
+	if (s->npass == s->cap &&
+	    ph2_sched_reserve(s, s->cap ? 2 * s->cap : 64) < 0)
+		return -1;
 	s->len[s->npass] = len;
 	s->tile_bits[s->npass] = tb;
+	s->low_bits[s->npass] = low;
 	s->npass++;
 	return 0;
 }
@@ -49,7 +63,6 @@ int ph2_sched_greedy(const struct ph2hip_rot *rot, size_t n, uint32_t nqb,
 		tile_bits = nqb;
 	if (low_bits > tile_bits)
 		low_bits = tile_bits;
-	out->low_bits = low_bits;
 	const uint64_t lowmask = (UINT64_C(1) << low_bits) - 1;
 	const uint32_t gmax = tile_bits - low_bits;
 	uint64_t basis[64];
@@ -62,7 +75,7 @@ int ph2_sched_greedy(const struct ph2hip_rot *rot, size_t n, uint32_t nqb,
 				v ^= basis[b];
 		const int grows = v != 0;
 		if (len == PH2HIP_PASS_MAX || (grows && g == gmax)) {
-			if (sched_push(out, len, tile_bits) < 0)
+			if (ph2_sched_push(out, len, tile_bits, low_bits) < 0)
 				return -1;
 			g = 0;
 			len = 0;
@@ -79,7 +92,7 @@ int ph2_sched_greedy(const struct ph2hip_rot *rot, size_t n, uint32_t nqb,
 		}
 		len++;
 	}
-	if (len && sched_push(out, len, tile_bits) < 0)
+	if (len && ph2_sched_push(out, len, tile_bits, low_bits) < 0)
 		return -1;
 	return 0;
 }

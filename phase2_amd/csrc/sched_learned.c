@@ -14,7 +14,7 @@
 #define CACHE_SLOTS 4096
 
 struct cand {
-	uint32_t i, j, tb;
+	uint32_t i, j, tb, low;
 };
 
 struct entry {
@@ -34,6 +34,7 @@ static struct {
 } LM;
 
 static const uint32_t TILES[] = { 11, 12 };
+#define NLOW 3	/* contiguous low bits low, low+1, low+2 */
 
 void ph2_learned_features(const struct ph2hip_rot *rot, size_t n,
 	uint32_t tile_bits, const uint32_t *shape, float *f)
@@ -77,22 +78,14 @@ static void cache_clear(void)
 
 static int sched_copy(struct ph2_sched *dst, const struct ph2_sched *src)
 {
-	if (dst->cap < src->npass) {
-		uint32_t *a = realloc(dst->len, sizeof *a * src->npass);
-		if (!a)
-			return -1;
-		dst->len = a;
-		uint32_t *b = realloc(dst->tile_bits, sizeof *b * src->npass);
-		if (!b)
-			return -1;
-		dst->tile_bits = b;
-		dst->cap = src->npass;
-	}
+	if (ph2_sched_reserve(dst, src->npass) < 0)
+		return -1;
 	memcpy(dst->len, src->len, sizeof *dst->len * src->npass);
 	memcpy(dst->tile_bits, src->tile_bits,
 		sizeof *dst->tile_bits * src->npass);
+	memcpy(dst->low_bits, src->low_bits,
+		sizeof *dst->low_bits * src->npass);
 	dst->npass = src->npass;
-	dst->low_bits = src->low_bits;
 	return 0;
 }
 
@@ -115,19 +108,26 @@ static inline int topbit(uint64_t v)
 	return 63 - __builtin_clzll(v);
 }
 
-/* Candidate passes from every start: the end just before each term that
- * widens the span, the longest feasible end, and the single term. */
+/* Candidate passes from every start, for every tile width and contiguous
+ * low-bit count: the end just before each term that widens the span, the
+ * longest feasible end, and the single term. */
 static size_t candidates(const struct ph2hip_rot *rot, size_t n,
 	uint32_t nqb, size_t cap)
 {
-	const uint64_t lowmask = (UINT64_C(1) << LM.low) - 1;
 	size_t m = 0;
 	for (size_t i = 0; i < n; i++) {
-		for (size_t t = 0; t < sizeof TILES / sizeof *TILES; t++) {
+		for (size_t c = 0; c < NLOW * sizeof TILES / sizeof *TILES; c++) {
+			const size_t t = c / NLOW;
+			const uint32_t dl = (uint32_t)(c % NLOW);
 			const uint32_t tb = TILES[t] < nqb ? TILES[t] : nqb;
 			if (tb < PH2HIP_TILE_MIN || (t > 0 && tb == TILES[t - 1]))
 				continue;
-			const uint32_t gmax = tb - (LM.low < tb ? LM.low : tb);
+			const uint32_t low = LM.low + dl < tb ? LM.low + dl : tb;
+			if (dl > 0 && low == (LM.low + dl - 1 < tb ?
+				    LM.low + dl - 1 : tb))
+				continue;
+			const uint64_t lowmask = (UINT64_C(1) << low) - 1;
+			const uint32_t gmax = tb - low;
 			uint64_t basis[64];
 			uint32_t g = 0;
 			size_t j = i;
@@ -141,7 +141,7 @@ static size_t candidates(const struct ph2hip_rot *rot, size_t n,
 						break;
 					if (j > i + 1 && m < cap)
 						LM.cand[m++] = (struct cand){ (uint32_t)i,
-							(uint32_t)j, tb };
+							(uint32_t)j, tb, low };
 					const int p = topbit(v);
 					for (uint32_t b = 0; b < g; b++)
 						if (basis[b] >> p & 1)
@@ -151,10 +151,10 @@ static size_t candidates(const struct ph2hip_rot *rot, size_t n,
 			}
 			if (j > i + 1 && m < cap)
 				LM.cand[m++] = (struct cand){ (uint32_t)i, (uint32_t)j,
-					tb };
+					tb, low };
 			if (m < cap)
 				LM.cand[m++] = (struct cand){ (uint32_t)i,
-					(uint32_t)(i + 1), tb };
+					(uint32_t)(i + 1), tb, low };
 		}
 	}
 	return m;
@@ -177,7 +177,7 @@ int ph2_learned_schedule(const struct ph2hip_rot *rot, size_t n, uint32_t nqb,
 		return sched_copy(out, &e->s);
 	}
 	LM.misses++;
-	const size_t cap = n * 2 * (MAXLEN / 4 + 3) + 16;
+	const size_t cap = n * 2 * NLOW * (MAXLEN / 4 + 3) + 16;
 	if (reserve(cap) < 0)
 		return -1;
 	size_t m = candidates(rot, n, nqb, cap);
@@ -186,7 +186,7 @@ int ph2_learned_schedule(const struct ph2hip_rot *rot, size_t n, uint32_t nqb,
 	for (size_t c = 0; c < m; c++) {
 		uint32_t shape[3];
 		const struct cand cd = LM.cand[c];
-		if (ph2hip_pass_shape(rot + cd.i, cd.j - cd.i, cd.tb, LM.low,
+		if (ph2hip_pass_shape(rot + cd.i, cd.j - cd.i, cd.tb, cd.low,
 			    shape) < 0)
 			continue;
 		LM.cand[k] = cd;
@@ -225,17 +225,15 @@ int ph2_learned_schedule(const struct ph2hip_rot *rot, size_t n, uint32_t nqb,
 		for (size_t j = n; j > 0; j = LM.cand[from[j]].i)
 			np++;
 		struct ph2_sched s = { 0 };
-		s.len = malloc(sizeof(uint32_t) * np);
-		s.tile_bits = malloc(sizeof(uint32_t) * np);
-		if (s.len && s.tile_bits) {
-			s.npass = s.cap = np;
-			s.low_bits = LM.low;
+		if (ph2_sched_reserve(&s, np) == 0) {
+			s.npass = np;
 			size_t p = np;
 			for (size_t j = n; j > 0; j = LM.cand[from[j]].i) {
 				const struct cand cd = LM.cand[from[j]];
 				p--;
 				s.len[p] = cd.j - cd.i;
 				s.tile_bits[p] = cd.tb;
+				s.low_bits[p] = cd.low;
 			}
 			rc = sched_copy(out, &s);
 			if (rc == 0) {
@@ -266,7 +264,7 @@ int ph2_learned_predict(const struct ph2hip_rot *rot, size_t n, uint32_t nqb,
 		uint32_t shape[3];
 		if (k0 + s->len[p] > n ||
 		    ph2hip_pass_shape(rot + k0, s->len[p], s->tile_bits[p],
-			    s->low_bits, shape) < 0)
+			    s->low_bits[p], shape) < 0)
 			return -1;
 		ph2_learned_features(rot + k0, s->len[p], s->tile_bits[p], shape,
 			LM.feat + p * PH2_LM_NIN);

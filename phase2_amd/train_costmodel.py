@@ -1,24 +1,25 @@
 """Record fused-pass timings, train the cost model on the GPU, evaluate it.
 
-    python -m phase2_amd.train_costmodel --out DIR [--extra-records GLOB]
+    python -m phase2_amd.train_costmodel --out DIR [--no-eval]
+    python -m phase2_amd.train_costmodel --out DIR --eval-only PARAMS
 
-1. Records: random cuts (phase2_amd.costmodel.random_schedule) of 1024-term
-   blocks of lexicographically sorted synthetic Hamiltonians at 28, 30 and
-   33 qubits, timed per pass with HIP events (ph2hip_prof_records), plus any
-   record files given with --extra-records.
+1. Records: random cuts (phase2_amd.costmodel.random_schedule: random pass
+   length, tile width 11/12 and 3..5 contiguous low bits per pass) of
+   1024-term blocks of lexicographically sorted synthetic Hamiltonians at 28
+   (two seeds), 30 and 33 qubits, timed per pass with HIP events
+   (ph2hip_prof_records), plus any record files given with --extra-records.
 2. Training: the library's FP32 network (85 -> 64 -> 64 -> 1), Adam, on the
    GPU, 10% held out.
 3. Evaluation on 33-qubit blocks never used for training: the same blocks
    are swept with (a) the greedy cut at tile 12 / low 4, (b) the greedy cut
-   at five tile/low settings around the default, keeping the fastest
-   (exhaustive tuning of the fixed setting), and (c) the learned schedule;
-   wall time per block after a synchronisation, and predicted-vs-measured
-   error per pass.
+   at four tile/low settings, keeping the fastest (exhaustive tuning of the
+   fixed setting), and (c) the learned schedule; wall time per block after
+   a synchronisation, and predicted-vs-measured error per pass.
 4. Evaluation on the composite integrator at 30 qubits (synth.cmpsit_n30):
    greedy, exhaustive and learned schedules over whole cmpsit steps, with
    the overlaps compared bit for bit.
-Writes costmodel_params.npz, records.npy, report.json and
-report_cmpsit.json into DIR (--eval-only PARAMS runs step 4 alone).
+Writes costmodel_params.npz, records.npy and train.json (steps 1-2), and
+report.json and report_cmpsit.json (steps 3-4) into DIR.
 """
 
 from __future__ import annotations
@@ -90,7 +91,7 @@ def eval_cmpsit(params, steps=1, log=log):
     res = {}
     outs = {}
     best = (None, 1e300)
-    for tb, low in ((12, 4), (12, 3), (12, 5), (11, 4), (11, 5)):
+    for tb, low in ((12, 4), (12, 3), (12, 5), (11, 4)):
         nat.set_default(nat.FUSED, tb, low)
         outs[(tb, low)], t = run()
         res[f"greedy_{tb}_{low}_s"] = t
@@ -98,7 +99,7 @@ def eval_cmpsit(params, steps=1, log=log):
         if t < best[1]:
             best = ((tb, low), t)
     nat.set_default(nat.FUSED, 12, 4)
-    cm.install(params, low=4, min_qb=28)
+    cm.install(params, min_qb=28)
     nat.prof_enable(True)
     nat.prof_records(reset=True)
     out_l, t_l = run()
@@ -126,17 +127,79 @@ def eval_cmpsit(params, steps=1, log=log):
     return res
 
 
+EVAL_BLOCKS = [9, 27, 44, 61, 78, 95]
+
+
+def eval_n33(params, log=log):
+    """Held-out 33-qubit sweep blocks: greedy 12/4, greedy at four tile/low
+    settings (exhaustive tuning of the fixed setting), learned."""
+    w = synth.trott_n33()
+    xm33, zm33, phi33 = sorted_terms(w["hm"])
+    reg = nat.Register(33, nat.FUSED, 12, 4)
+    reg.prep_multidet(w["det_idx"], w["det_cf"])
+    res = {}
+    best = (None, 1e300)
+    for tb, low in ((12, 4), (12, 3), (12, 5), (11, 4)):
+        reg.configure(nat.FUSED, tb, low)
+        t = time_blocks(reg, xm33, zm33, phi33, EVAL_BLOCKS,
+                        f"greedy {tb}/{low}")
+        res[f"greedy_{tb}_{low}_s"] = t
+        if sum(t) < best[1]:
+            best = ((tb, low), sum(t))
+    reg.configure(nat.FUSED, 12, 4)
+    cm.install(params, min_qb=28)
+    # first sweep computes and caches the schedules, the second is timed
+    t_first = time_blocks(reg, xm33, zm33, phi33, EVAL_BLOCKS,
+                          "learned, first visit")
+    nat.prof_enable(True)
+    nat.prof_records(reset=True)
+    t_learned = time_blocks(reg, xm33, zm33, phi33, EVAL_BLOCKS, "learned")
+    lrec = nat.prof_records(reset=True)
+    nat.prof_enable(False)
+    ov_l = reg.overlap(w["det_idx"], w["det_cf"])
+    hits, misses = cm.stats()
+    cm.uninstall()
+    reg.free()
+    Xl, yl, lrec = cm.features_from_records(lrec)
+    m = cm.MLP()
+    m.set_params(params)
+    pl = m.forward(Xl)
+    m.close()
+    err = np.abs(np.exp(pl - yl) - 1.0)
+    g = sum(res["greedy_12_4_s"])
+    res.update(
+        learned_s=t_learned, learned_first_visit_s=t_first,
+        greedy_total_s=g, learned_total_s=float(sum(t_learned)),
+        learned_vs_greedy=float(g / sum(t_learned)),
+        exhaustive_best=dict(tile=best[0][0], low=best[0][1],
+                             total_s=best[1]),
+        learned_vs_exhaustive=float(best[1] / sum(t_learned)),
+        eval_blocks=EVAL_BLOCKS,
+        learned_passes=int(len(lrec)),
+        learned_low_bits=np.bincount(lrec["low_bits"], minlength=13
+                                     ).tolist(),
+        learned_tile_bits=np.bincount(lrec["tile_bits"], minlength=13
+                                      ).tolist(),
+        eval_pass_median_rel_err=float(np.median(err)),
+        eval_pass_p90_rel_err=float(np.quantile(err, 0.9)),
+        cache_hits=hits, cache_misses=misses,
+        overlap_after_learned=[ov_l.real, ov_l.imag])
+    return res
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
     ap.add_argument("--extra-records", default="")
-    ap.add_argument("--blocks28", type=int, default=60)
-    ap.add_argument("--blocks30", type=int, default=36)
-    ap.add_argument("--blocks33", type=int, default=10)
-    ap.add_argument("--epochs", type=int, default=250)
+    ap.add_argument("--blocks28", type=int, default=75)
+    ap.add_argument("--blocks30", type=int, default=60)
+    ap.add_argument("--blocks33", type=int, default=12)
+    ap.add_argument("--epochs", type=int, default=300)
     ap.add_argument("--eval-only", default="",
-                    help="parameter file: skip collection and training, "
-                    "evaluate on the composite integrator only")
+                    help="parameter file: skip collection and training and "
+                    "evaluate (held-out 33-qubit blocks, composite "
+                    "integrator at 30 qubits)")
+    ap.add_argument("--no-eval", action="store_true")
     ap.add_argument("--cmpsit-steps", type=int, default=1)
     args = ap.parse_args(argv)
     out = Path(args.out)
@@ -144,31 +207,38 @@ def main(argv=None):
     nat.init()
     if args.eval_only:
         params, _ = cm.load(args.eval_only)
-        res = eval_cmpsit(params, args.cmpsit_steps)
-        (out / "report_cmpsit.json").write_text(json.dumps(res, indent=1))
+        res = eval_n33(params)
+        (out / "report.json").write_text(json.dumps(res, indent=1))
         log(json.dumps(res))
+        res_c = eval_cmpsit(params, args.cmpsit_steps)
+        (out / "report_cmpsit.json").write_text(json.dumps(res_c, indent=1))
+        log(json.dumps(res_c))
         return
     rng = np.random.default_rng(2024)
 
     recs = []
-    # 28 and 30 qubits: own Hamiltonians of the same generator
-    for nqb, nb in ((28, args.blocks28), (30, args.blocks30)):
-        hm = synth.hamiltonian(nqb, 0, seed=nqb, decay=True,
+    # 28 qubits (two Hamiltonians: one holds at most ~80 blocks of distinct
+    # four-index strings) and 30 qubits, same generator
+    for nqb, nb, seed in ((28, args.blocks28, 28), (28, args.blocks28, 128),
+                          (30, args.blocks30, 30)):
+        hm = synth.hamiltonian(nqb, 0, seed=seed, decay=True,
                                n_total=1024 * (nb + 4))
         blocks = list(range(2, nb + 2))
         recs.append(collect_width(nqb, hm, blocks, rng))
-        log(f"records n={nqb}: {len(recs[-1])}")
+        log(f"records n={nqb} seed {seed}: {len(recs[-1])}")
     # 33 qubits: blocks of the headline sweep, held-out blocks excluded
     w = synth.trott_n33()
     xm33, zm33, phi33 = sorted_terms(w["hm"])
     nblk = (len(phi33) + 1023) // 1024
-    eval_blocks = [9, 27, 44, 61, 78, 95]
-    train33 = [b for b in np.linspace(3, nblk - 2, args.blocks33).astype(int)
-               if b not in eval_blocks]
+    train33 = [int(b) for b in np.linspace(3, nblk - 2, args.blocks33
+                                           ).astype(int)
+               if b not in EVAL_BLOCKS]
     reg = nat.Register(33, nat.FUSED, 12, 4)
     reg.prep_multidet(w["det_idx"], w["det_cf"])
     recs.append(cm.collect(reg, xm33, zm33, phi33, train33, rng,
                            log=lambda m: log(f"n=33 {m}")))
+    log(f"records n=33: {len(recs[-1])}")
+    reg.free()
     for f in sorted(glob.glob(args.extra_records)) if args.extra_records else []:
         r = np.load(f, allow_pickle=False)
         recs.append(r[r["kind"] == 2])
@@ -187,57 +257,17 @@ def main(argv=None):
     pv = m.forward(X[va])
     rel_val = float(np.median(np.abs(np.exp(pv - y[va]) - 1.0)))
     m.close()
-    cm.save(params, out / "costmodel_params.npz")
-
-    # evaluation on held-out 33-qubit blocks
-    res = {}
-    t_greedy = time_blocks(reg, xm33, zm33, phi33, eval_blocks, "greedy 12/4")
-    res["greedy_12_4_s"] = t_greedy
-    best = (None, 1e300)
-    for tb, low in ((12, 3), (12, 4), (12, 5), (11, 4), (11, 5)):
-        reg.configure(nat.FUSED, tb, low)
-        t = sum(time_blocks(reg, xm33, zm33, phi33, eval_blocks,
-                            f"greedy {tb}/{low}"))
-        res[f"greedy_{tb}_{low}_total_s"] = t
-        if t < best[1]:
-            best = ((tb, low), t)
-    res["exhaustive_best"] = dict(tile=best[0][0], low=best[0][1],
-                                  total_s=best[1])
-    reg.configure(nat.FUSED, 12, 4)
-    cm.install(params, low=4, min_qb=28)
-    # first sweep computes and caches the schedules, the second is timed
-    time_blocks(reg, xm33, zm33, phi33, eval_blocks, "learned, first visit")
-    nat.prof_enable(True)
-    nat.prof_records(reset=True)
-    t_learned = time_blocks(reg, xm33, zm33, phi33, eval_blocks, "learned")
-    lrec = nat.prof_records(reset=True)
-    nat.prof_enable(False)
-    Xl, yl, lrec = cm.features_from_records(lrec)
-    m = cm.MLP()
-    m.set_params(params)
-    pl = m.forward(Xl)
-    m.close()
-    err = np.abs(np.exp(pl - yl) - 1.0)
-    hits, misses = cm.stats()
-    cm.uninstall()
-    reg.free()
-    res.update(
-        learned_s=t_learned,
-        greedy_total_s=float(sum(t_greedy)),
-        learned_total_s=float(sum(t_learned)),
-        learned_vs_greedy=float(sum(t_greedy) / sum(t_learned)),
-        learned_vs_exhaustive=float(best[1] / sum(t_learned)),
-        eval_blocks=eval_blocks,
-        records=int(len(y)),
-        train_seconds=train_s,
-        val_median_rel_err=rel_val,
-        eval_pass_median_rel_err=float(np.median(err)),
-        eval_pass_p90_rel_err=float(np.quantile(err, 0.9)),
-        eval_passes=int(len(err)),
-        cache_hits=hits, cache_misses=misses,
-        history=info["history"])
-    (out / "report.json").write_text(json.dumps(res, indent=1))
+    cm.save(params, out / "costmodel_params.npz", records=len(y))
+    res = dict(records=int(len(y)), train_seconds=train_s,
+               val_median_rel_err=rel_val, history=info["history"],
+               train33_blocks=train33)
+    (out / "train.json").write_text(json.dumps(res, indent=1))
     log(json.dumps({k: v for k, v in res.items() if k != "history"}))
+    if args.no_eval:
+        return
+    res_n = eval_n33(params)
+    (out / "report.json").write_text(json.dumps(res_n, indent=1))
+    log(json.dumps(res_n))
     res_c = eval_cmpsit(params, args.cmpsit_steps)
     (out / "report_cmpsit.json").write_text(json.dumps(res_c, indent=1))
     log(json.dumps(res_c))
