@@ -577,7 +577,7 @@ def main(argv=None):
     if args.gpus != 1 or world != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, "
               f"WORLD_SIZE={world}): the register lives on a single MI355X "
-              "and there is no multi-GPU path in this build.",
+              "and the library has no multi-GPU path.",
               file=sys.stderr, flush=True)
         sys.exit(2)
     if args.steps is None:
