@@ -90,6 +90,12 @@ int ph2hip_rotate_fused(void *amp, uint32_t nqb, const struct ph2hip_rot *rot,
  * workgroup per tile.  Initial mask from PHASE2_HIP_PIPE, else 0. */
 int ph2hip_set_fused_form(uint32_t pipe_mask, uint64_t persistent_wgs);
 
+/* Whether the first and last micro-group of a fused pass may move their
+ * cosets straight between HBM and registers (when their lanes cover whole
+ * 256-byte runs), skipping the LDS staging of the tile; on by default.
+ * Every setting gives the same state. */
+int ph2hip_set_fused_direct(int on);
+
 
 /* Shape of the fused pass over rot[0..n) without launching it: shape[0] =
  * contiguous low bits L, shape[1] = gathered dimensions g, shape[2] =
@@ -184,14 +190,15 @@ int ph2hip_prof_read(struct ph2hip_prof *out, int reset);
 
 /* Per-launch records kept while profiling is enabled (the newest
  * PH2HIP_REC_MAX).  For fused passes: tile bits, contiguous low bits L,
- * gathered dimensions g, terms, micro-groups, the index bits the tile
- * gathers (pivot mask), the union of the pass's flip and phase masks, and
- * the kernel time.  ph2hip_prof_records copies up to cap records (oldest
+ * gathered dimensions g, terms, micro-groups, which of the first (bit 0)
+ * and last (bit 1) micro-groups move their cosets straight between HBM and
+ * registers, the index bits the tile gathers (pivot mask), the union of the
+ * pass's flip and phase masks, and the kernel time.  ph2hip_prof_records copies up to cap records (oldest
  * first, after draining pending events) and returns how many, or -1. */
 #define PH2HIP_REC_MAX 65536
 struct ph2hip_launch_rec {
 	uint32_t kind, nqb, tile_bits, low_bits;
-	uint32_t gdim, nterms, ngroups, pad;
+	uint32_t gdim, nterms, ngroups, direct;
 	uint64_t pivots, xunion, zunion;
 	double ms;
 };

@@ -338,7 +338,7 @@ class Register:
 
 REC_DTYPE = np.dtype([
     ("kind", "<u4"), ("nqb", "<u4"), ("tile_bits", "<u4"), ("low_bits", "<u4"),
-    ("gdim", "<u4"), ("nterms", "<u4"), ("ngroups", "<u4"), ("pad", "<u4"),
+    ("gdim", "<u4"), ("nterms", "<u4"), ("ngroups", "<u4"), ("direct", "<u4"),
     ("pivots", "<u8"), ("xunion", "<u8"), ("zunion", "<u8"), ("ms", "<f8")])
 
 
@@ -359,6 +359,12 @@ def set_fused_form(pipe_mask: int, persistent_wgs: int = 0):
     if lib().ph2hip_set_fused_form(C.c_uint32(pipe_mask),
                                    C.c_uint64(persistent_wgs)) != 0:
         _err("ph2hip_set_fused_form")
+
+
+def set_fused_direct(on: bool):
+    """First/last micro-groups straight between HBM and registers."""
+    if lib().ph2hip_set_fused_direct(C.c_int(int(on))) != 0:
+        _err("ph2hip_set_fused_direct")
 
 
 def set_plain_form(form: int):

@@ -276,6 +276,10 @@ constexpr int kMaxPer = (1 << PH2HIP_TILE_MAX) / kBlock;
 struct FusedPass {
 	uint64_t w[PH2HIP_TILE_MAX];   // gathered basis vectors
 	uint64_t roff[kMaxPer];        // gathered offset of load/store step m
+	// Offset (row part ^ column part) of coset element w of the first
+	// ([0]) and last ([1]) micro-group, for groups that move their cosets
+	// straight between HBM and registers (dflags).
+	uint64_t dcw[2][16];
 	uint32_t piv[PH2HIP_TILE_MAX]; // their leading bits, ascending
 	union {
 		struct {
@@ -283,6 +287,9 @@ struct FusedPass {
 		};
 		uint32_t hdr;
 	};
+	// bit 0: the first group loads its cosets from HBM (the tile is not
+	// staged in LDS first); bit 1: the last group stores them to HBM.
+	uint32_t dflags;
 	FusedGroup grp[PH2HIP_PASS_MAX];
 	FusedTerm t[PH2HIP_PASS_MAX];
 };
@@ -381,18 +388,21 @@ __device__ __forceinline__ uint64_t tile_base(const FusedPass &P, uint64_t t)
 	return base;
 }
 
-// Persistent, software-pipelined pass: each workgroup walks tiles
-// blockIdx.x, +gridDim.x, ...; the next tile's HBM reads are in flight in
-// registers while the current tile is rotated in LDS, and its writes are
-// issued without waiting.
-// All micro-groups of a pass on one tile held in LDS.
+// All micro-groups of a pass on one tile.  Each group's cosets come from
+// and go back to the LDS tile, except that with dflags bit 0 the first
+// group reads its cosets straight from HBM (the tile was not staged) and
+// with bit 1 the last group writes them straight to HBM (the tile is not
+// stored from LDS afterwards).  Those groups map coset-index bits 0..3 to
+// tile bits 0..3 (checked on the host, with L >= 4), so 16 lanes move one
+// 256-byte run; a pass of one group then never touches LDS.
 template <int TB, int D>
-__device__ __forceinline__ void tile_groups(v2d *tile, const FusedPass &P,
-	uint64_t base, uint32_t tid)
+__device__ __forceinline__ void tile_groups(v2d *__restrict__ a, v2d *tile,
+	const FusedPass &P, uint64_t base, uint32_t tid, uint32_t dflags)
 {
 	constexpr uint32_t NSUB = (1u << TB) >> D;    // cosets per group
 	static_assert(NSUB % kBlock == 0, "cosets must cover the block");
 	const uint32_t ngroups = pass_ngroups(P);
+	const uint32_t L = pass_L(P), g = pass_g(P), lmask = (1u << L) - 1;
 	for (uint32_t gi = 0; gi < ngroups; gi++) {
 		const uint32_t uw0 = P.grp[gi].uw[0], uw1 = P.grp[gi].uw[1];
 		const uint32_t jw0 = P.grp[gi].jposw[0], jw1 = P.grp[gi].jposw[1],
@@ -401,7 +411,10 @@ __device__ __forceinline__ void tile_groups(v2d *tile, const FusedPass &P,
 		const uint32_t gu[4] = { uw0 & 0xffffu, uw0 >> 16, uw1 & 0xffffu,
 			uw1 >> 16 };
 		const uint32_t jw[3] = { jw0, jw1, jw2 };
-		__syncthreads();
+		const bool gload = gi == 0 && (dflags & 1u);
+		const bool gstore = gi + 1 == ngroups && (dflags & 2u);
+		if (!gload)
+			__syncthreads();
 		for (uint32_t j = tid; j < NSUB; j += kBlock) {
 			// coset base: j's bits spread over the free tile bits
 			uint32_t bu = 0;
@@ -410,17 +423,34 @@ __device__ __forceinline__ void tile_groups(v2d *tile, const FusedPass &P,
 				bu |= (j >> k & 1u) <<
 					((jw[k >> 2] >> (8 * (k & 3))) & 0xffu);
 			const uint32_t sbu = lds_slot(bu);
+			// HBM index of coordinate bu (row part ^ column part)
+			uint64_t abu = 0;
+			if (gload || gstore) {
+				abu = base ^ (bu & lmask);
+				for (uint32_t b = 0; b < g; b++)
+					if ((bu >> (L + b)) & 1u)
+						abu ^= P.w[b];
+			}
 			double re[1 << D], im[1 << D];
+			if (gload) {
 #pragma unroll
-			for (int w = 0; w < (1 << D); w++) {
-				uint32_t cw = 0;
+				for (int w = 0; w < (1 << D); w++) {
+					const v2d v = a[abu ^ P.dcw[0][w]];
+					re[w] = v.x;
+					im[w] = v.y;
+				}
+			} else {
 #pragma unroll
-				for (int i = 0; i < D; i++)
-					if (w >> i & 1)
-						cw ^= gu[i];
-				const v2d v = tile[sbu ^ lds_slot(cw)];
-				re[w] = v.x;
-				im[w] = v.y;
+				for (int w = 0; w < (1 << D); w++) {
+					uint32_t cw = 0;
+#pragma unroll
+					for (int i = 0; i < D; i++)
+						if (w >> i & 1)
+							cw ^= gu[i];
+					const v2d v = tile[sbu ^ lds_slot(cw)];
+					re[w] = v.x;
+					im[w] = v.y;
+				}
 			}
 			const uint32_t kfirst = fcw & 0xffu;
 			const uint32_t kend = kfirst + ((fcw >> 8) & 0xffu);
@@ -458,21 +488,32 @@ __device__ __forceinline__ void tile_groups(v2d *tile, const FusedPass &P,
 							ztp, sb, c);
 				}
 			}
+			if (gstore) {
 #pragma unroll
-			for (int w = 0; w < (1 << D); w++) {
-				uint32_t cw = 0;
+				for (int w = 0; w < (1 << D); w++) {
+					v2d v;
+					v.x = re[w];
+					v.y = im[w];
+					a[abu ^ P.dcw[1][w]] = v;
+				}
+			} else {
 #pragma unroll
-				for (int i = 0; i < D; i++)
-					if (w >> i & 1)
-						cw ^= gu[i];
-				v2d v;
-				v.x = re[w];
-				v.y = im[w];
-				tile[sbu ^ lds_slot(cw)] = v;
+				for (int w = 0; w < (1 << D); w++) {
+					uint32_t cw = 0;
+#pragma unroll
+					for (int i = 0; i < D; i++)
+						if (w >> i & 1)
+							cw ^= gu[i];
+					v2d v;
+					v.x = re[w];
+					v.y = im[w];
+					tile[sbu ^ lds_slot(cw)] = v;
+				}
 			}
 		}
 	}
-	__syncthreads();
+	if (!(dflags & 2u))
+		__syncthreads();
 }
 
 // Direct HBM->LDS tile loads (gfx950 global_load_lds_dwordx4): the data
@@ -542,7 +583,10 @@ __device__ __forceinline__ void fused_tile(v2d *__restrict__ a,
 			wt ^= P.w[b];
 	const uint64_t base = tile_base(P, t);
 	const uint64_t bo = base ^ wt;
-	if (L >= 4) {
+	const uint32_t dflags = P.dflags;
+	if (dflags & 1u) {
+		// the first group reads its cosets from HBM itself
+	} else if (L >= 4) {
 		dma_tile<TB>(a, tile, P, bo, tid);
 		vm_wait<0>();
 	} else {
@@ -552,8 +596,9 @@ __device__ __forceinline__ void fused_tile(v2d *__restrict__ a,
 			tile[lds_slot(tid + m * kBlock)] =
 				a[bo ^ P.roff[m] ^ ((tid + m * kBlock) & lmask)];
 	}
-	tile_groups<TB, D>(tile, P, base, tid);
-	store_tile<TB>(a, tile, P, bo, tid);
+	tile_groups<TB, D>(a, tile, P, base, tid, dflags);
+	if (!(dflags & 2u))
+		store_tile<TB>(a, tile, P, bo, tid);
 }
 
 // One tile per workgroup (grid = number of tiles); HBM latency is hidden
@@ -605,6 +650,10 @@ __global__ __launch_bounds__(kBlock) void k_rot_fused_batch(
 // Persistent, software-pipelined form: each workgroup walks tiles
 // blockIdx.x, +gridDim.x, ...; the next tile's HBM reads are in flight in
 // registers while the current tile is rotated in LDS.
+// Persistent, software-pipelined pass: each workgroup walks tiles
+// blockIdx.x, +gridDim.x, ...; the next tile's HBM reads are in flight in
+// registers while the current tile is rotated in LDS, and its writes are
+// issued without waiting.  The groups always go through LDS here.
 template <int TB, int D>
 __global__ __launch_bounds__(kBlock) void k_rot_fused_pipe(
 	double2 *__restrict__ amp, const FusedPass P, uint64_t ntiles)
@@ -640,7 +689,7 @@ __global__ __launch_bounds__(kBlock) void k_rot_fused_pipe(
 				pre[m] = a[bn ^ P.roff[m] ^
 					((tid + m * kBlock) & lmask)];
 		}
-		tile_groups<TB, D>(tile, P, base, tid);
+		tile_groups<TB, D>(a, tile, P, base, tid, 0u);
 		const uint64_t bo = base ^ wt;
 #pragma unroll
 		for (uint32_t m = 0; m < PER; m++)
@@ -936,6 +985,7 @@ constexpr int group_depth(int tb)
 
 bool g_pipe_init = false;
 uint32_t g_pipe_mask = 0;   // bit TB set: persistent pipelined form
+bool g_direct = true;       // first/last groups straight to and from HBM
 
 template <int TB>
 int launch_fused(double2 *a, uint32_t nqb, const FusedPass &P,
@@ -954,6 +1004,7 @@ int launch_fused(double2 *a, uint32_t nqb, const FusedPass &P,
 	meta.low_bits = P.L;
 	meta.gdim = P.g;
 	meta.ngroups = P.ngroups;
+	meta.direct = P.dflags;
 	for (uint32_t b = 0; b < P.g; b++)
 		meta.pivots |= UINT64_C(1) << P.piv[b];
 	meta.xunion = xunion;
@@ -1098,8 +1149,10 @@ int build_pass(FusedPass &P, const struct ph2hip_rot *r, size_t len,
 		while (e < len && reduce32(xt + k, e + 1 - k, u, D) <= D)
 			e++;
 		uint32_t d = reduce32(xt + k, e - k, u, D);
-		// pad with unit vectors of tile bits outside the span
-		for (uint32_t bit = 0; d < D && bit < tb; bit++) {
+		// pad with unit vectors of tile bits outside the span, highest
+		// first, so that tile bits 0..3 stay free for the lanes where
+		// possible (whole 256-byte runs per quarter-wave)
+		for (uint32_t bit = tb; d < D && bit-- > 0;) {
 			uint32_t x = 1u << bit;
 			for (uint32_t i = 0; i < d && x; i++)
 				if (x >> (31 - __builtin_clz(u[i])) & 1)
@@ -1168,6 +1221,28 @@ int build_pass(FusedPass &P, const struct ph2hip_rot *r, size_t len,
 		k = e;
 	}
 	P.ngroups = (uint8_t)ng;
+	// First and last group straight between HBM and registers when their
+	// coset-index bits 0..3 are tile bits 0..3 inside rows of >= 16.
+	if (g_direct && L >= 4) {
+		for (int side = 0; side < 2; side++) {
+			const FusedGroup &G = P.grp[side ? ng - 1 : 0];
+			if (G.jpos[0] != 0 || G.jpos[1] != 1 || G.jpos[2] != 2 ||
+			    G.jpos[3] != 3)
+				continue;
+			for (uint32_t e = 0; e < (1u << D); e++) {
+				uint32_t cw = 0;
+				for (uint32_t i = 0; i < D; i++)
+					if (e >> i & 1)
+						cw ^= G.u[i];
+				uint64_t off = cw & lm;
+				for (uint32_t b = 0; b < g; b++)
+					if ((cw >> (L + b)) & 1)
+						off ^= w[b];
+				P.dcw[side][e] = off;
+			}
+			P.dflags |= 1u << side;
+		}
+	}
 	return 0;
 }
 
@@ -1646,6 +1721,12 @@ int ph2hip_set_fused_form(uint32_t pipe_mask, uint64_t persistent_wgs)
 	g_pipe_init = true;
 	if (persistent_wgs)
 		g_persist_wgs = persistent_wgs;
+	return 0;
+}
+
+int ph2hip_set_fused_direct(int on)
+{
+	g_direct = on != 0;
 	return 0;
 }
 

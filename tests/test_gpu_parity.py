@@ -303,8 +303,9 @@ def test_fused_pipelined_form_bit_identical(oracle, pipe_tile):
 def test_fused_direct_load_bit_identical(oracle, tile, low):
     """Tiles with at least 4 contiguous low bits are loaded straight into LDS
     (lanes permuted within 256-byte runs to match the LDS swizzle), the
-    others through registers; both leave the state bit-identical to the
-    oracle."""
+    others through registers; with the direct form on, eligible first and
+    last micro-groups move their cosets between HBM and registers instead.
+    Every combination leaves the state bit-identical to the oracle."""
     n = 18
     rng = np.random.default_rng(60 + tile + low)
     st = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
@@ -313,8 +314,13 @@ def test_fused_direct_load_bit_identical(oracle, tile, low):
     xm, zm = hm.xm[perm], hm.zm[perm]
     phi = 0.3 * hm.cf[perm] / np.max(np.abs(hm.cf))
     want = oracle.rotate(st, xm, zm, phi)
-    got = nat.rotate(st, xm, zm, phi, nat.FUSED, tile, low)
-    assert np.array_equal(got, want)
+    for direct in (True, False):
+        try:
+            nat.set_fused_direct(direct)
+            got = nat.rotate(st, xm, zm, phi, nat.FUSED, tile, low)
+        finally:
+            nat.set_fused_direct(True)
+        assert np.array_equal(got, want), direct
 
 
 def _batch_regs(widths, depth=120, samples=2):
