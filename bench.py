@@ -427,8 +427,8 @@ def bench_trott2_n26(args, nat, costmodel, schedule):
                 "determinants": 230230},
         extra={"terms_per_pass": kp["terms"] / max(1, kp["launches"])},
         cpu=lambda ranks: cpu_rotations(
-            26, xm[:128], zm[:128], 0.5 * w["delta"] * cf[:128], ranks,
-            "first 128 rotations of a trott2 forward half-sweep"),
+            26, xm[:256], zm[:256], 0.5 * w["delta"] * cf[:256], ranks,
+            "first 256 rotations of a trott2 forward half-sweep"),
         nqb=26)
 
 
@@ -520,7 +520,7 @@ def bench_qdrift_batch(args, nat, costmodel, schedule):
 def cpu_qdrift_sample(regs, ranks):
     """Reference qdrift_simul on the first registers of 20..22 qubits of
     the batch, one sample each."""
-    pick = [r for r in regs if 20 <= r["hm"].nqb <= 22][:3]
+    pick = [r for r in regs if 20 <= r["hm"].nqb <= 22][:6]
     tot_upd = tot_s = 0.0
     res = None
     for r in pick:
@@ -569,7 +569,7 @@ def main(argv=None):
                     default=True)
     ap.add_argument("--no-cpu-baseline", dest="cpu", action="store_false")
     ap.add_argument("--cpu-nqb", type=int, default=28)
-    ap.add_argument("--cpu-terms", type=int, default=16)
+    ap.add_argument("--cpu-terms", type=int, default=64)
     ap.add_argument("--cpu-ranks", type=int, default=0)
     args = ap.parse_args(argv)
 
