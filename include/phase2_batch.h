@@ -15,7 +15,9 @@
  *   PH2_BATCH_PLAIN_EACH    one plain launch per rotation per register,
  *   PH2_BATCH_FUSED_EACH    fused passes, launched register by register,
  *   PH2_BATCH_FUSED_BATCHED fused passes of all registers in one launch per
- *                           pass index, the round captured in a hipGraph.
+ *                           pass index, the round captured in a hipGraph;
+ *                           the host draws and plans round s+1 while the
+ *                           device runs round s.
  * tile_bits == 0 asks the autotuner to pick tile and low bits by timing
  * candidates on the first round of a subset of the registers.
  */
@@ -49,6 +51,8 @@ enum ph2_batch_mode {
 struct ph2_batch_stats {
 	double seconds;		/* whole run, tuning excluded */
 	double tune_seconds;
+	double prep_seconds;	/* host draws and pass building (overlaps the
+				 * device in the batched mode) */
 	double amp_updates;	/* sum over registers of samples*depth*2^nqb */
 	uint64_t launches;	/* rotation kernel launches */
 	uint64_t passes;	/* fused passes (sum over registers) */

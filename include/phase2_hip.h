@@ -133,6 +133,21 @@ int ph2hip_rotate_batched(void *amp, uint64_t namp_total,
 	const struct ph2hip_batch_job *jobs, size_t njobs, uint32_t tile_bits,
 	uint32_t low_bits, int use_graph, uint64_t *launches);
 
+/* The same in two halves, so that a caller can prepare the next batch on
+ * the host while the device runs this one.  plan_build validates the jobs
+ * and builds every pass descriptor on the host (no device work; the jobs'
+ * arrays may change once it returns); plan_launch uploads the plan and
+ * queues its launches (a hipGraph with use_graph) behind the work already
+ * on the library stream, and returns without waiting.  A plan may be freed
+ * after ph2hip_synchronize (or any call that waits for the stream). */
+struct ph2hip_batch_plan;
+int ph2hip_batch_plan_build(const struct ph2hip_batch_job *jobs, size_t njobs,
+	uint32_t tile_bits, uint32_t low_bits, uint64_t namp_total,
+	struct ph2hip_batch_plan **plan);
+int ph2hip_batch_plan_launch(void *amp, struct ph2hip_batch_plan *plan,
+	int use_graph, uint64_t *launches);
+void ph2hip_batch_plan_free(struct ph2hip_batch_plan *plan);
+
 /* amp[idx[k]] = (val[2k], val[2k+1]) with idx < namp_total (host arrays);
  * for repeated indices the last entry wins, as sequential qreg_setamp
  * calls would leave it. */

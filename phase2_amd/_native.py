@@ -385,7 +385,7 @@ class BatchReg(C.Structure):
 
 class BatchStats(C.Structure):
     _fields_ = [("seconds", C.c_double), ("tune_seconds", C.c_double),
-                ("amp_updates", C.c_double), ("launches", C.c_uint64),
+                ("prep_seconds", C.c_double), ("amp_updates", C.c_double), ("launches", C.c_uint64),
                 ("passes", C.c_uint64), ("tile_bits", C.c_uint32),
                 ("low_bits", C.c_uint32)]
 

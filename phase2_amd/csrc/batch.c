@@ -219,6 +219,32 @@ static int run_round(struct batch *B, size_t s, size_t lim, int mode,
 	return 0;
 }
 
+/* Host half of a batched round: greedy schedules of every active register
+ * and the pass descriptors of the whole round (no device work). */
+static int plan_round(struct batch *B, size_t s, uint32_t tb, uint32_t low,
+	struct ph2hip_batch_plan **plan)
+{
+	size_t nj = 0;
+	for (size_t i = 0; i < B->n; i++) {
+		struct breg *g = &B->b[i];
+		if (!active(g, s) || g->r->depth == 0)
+			continue;
+		const uint32_t nqb = g->r->nqb;
+		if (ph2_sched_greedy(g->rot, g->r->depth, nqb, tb, low, &g->sch) <
+			0)
+			return -1;
+		B->passes += g->sch.npass;
+		struct ph2hip_batch_job *J = &B->jobs[nj++];
+		J->amp_off = g->off;
+		J->nqb = nqb;
+		J->rot = g->rot;
+		J->n = g->r->depth;
+		J->pass_len = g->sch.len;
+		J->npass = g->sch.npass;
+	}
+	return ph2hip_batch_plan_build(B->jobs, nj, tb, low, B->namp, plan);
+}
+
 static int measure_round(struct batch *B, size_t s, double *out)
 {
 	size_t m = 0;
@@ -313,18 +339,61 @@ int ph2_batch_qdrift(const struct ph2_batch_reg *regs, size_t nreg, int mode,
 	}
 	if (ph2hip_synchronize() < 0)
 		rc = -1;
+	/* Batched rounds are pipelined: while the device runs round s, the
+	 * host draws round s+1 and builds its plan (the draws do not depend on
+	 * the state, and each register's stream stays in order). */
+	int pipelined = mode == PH2_BATCH_FUSED_BATCHED;
+	for (size_t i = 0; i < nreg && pipelined; i++)
+		if (regs[i].nqb < tile_bits)
+			pipelined = 0;   /* narrow registers take other paths */
+	double prep = 0.0;
 	const double t0 = now_s();
+	struct ph2hip_batch_plan *plan = NULL, *next = NULL;
+	if (pipelined && rc == 0 && rounds) {
+		const double tp = now_s();
+		rc = plan_round(&B, 0, tile_bits, low_bits, &next);
+		prep += now_s() - tp;
+	}
 	for (size_t s = 0; s < rounds && rc == 0; s++) {
-		if (s > 0)
-			for (size_t i = 0; i < nreg; i++)
-				if (active(&B.b[i], s))
-					draw(&B.b[i]);
+		if (!pipelined) {
+			const double tp = now_s();
+			if (s > 0)
+				for (size_t i = 0; i < nreg; i++)
+					if (active(&B.b[i], s))
+						draw(&B.b[i]);
+			prep += now_s() - tp;
+			rc = prep_round(&B, s, nreg);
+			if (!rc)
+				rc = run_round(&B, s, nreg, mode, tile_bits,
+					low_bits);
+			if (!rc)
+				rc = measure_round(&B, s, out);
+			continue;
+		}
+		plan = next;
+		next = NULL;
+		uint64_t nl = 0;
 		rc = prep_round(&B, s, nreg);
 		if (!rc)
-			rc = run_round(&B, s, nreg, mode, tile_bits, low_bits);
+			rc = ph2hip_batch_plan_launch(B.amp, plan, 1, &nl);
+		B.launches += nl;
+		if (!rc && s + 1 < rounds) {
+			const double tp = now_s();
+			for (size_t i = 0; i < nreg; i++)
+				if (active(&B.b[i], s + 1))
+					draw(&B.b[i]);
+			rc = plan_round(&B, s + 1, tile_bits, low_bits, &next);
+			prep += now_s() - tp;
+		}
 		if (!rc)
-			rc = measure_round(&B, s, out);
+			rc = measure_round(&B, s, out);   /* waits for round s */
+		ph2hip_batch_plan_free(plan);
+		plan = NULL;
 	}
+	if (ph2hip_synchronize() < 0)
+		rc = -1;
+	ph2hip_batch_plan_free(plan);
+	ph2hip_batch_plan_free(next);
 	if (rc == 0 && ph2hip_synchronize() < 0)
 		rc = -1;
 	if (rc < 0)
@@ -333,6 +402,7 @@ int ph2_batch_qdrift(const struct ph2_batch_reg *regs, size_t nreg, int mode,
 		memset(stats, 0, sizeof *stats);
 		stats->seconds = now_s() - t0;
 		stats->tune_seconds = tune;
+		stats->prep_seconds = prep;
 		for (size_t i = 0; i < nreg; i++)
 			stats->amp_updates += (double)regs[i].samples *
 				(double)regs[i].depth *

@@ -1776,12 +1776,38 @@ int ph2hip_norm2(const void *amp, uint32_t nqb, double *out)
 	return 0;
 }
 
-int ph2hip_rotate_batched(void *amp, uint64_t namp_total,
-	const struct ph2hip_batch_job *jobs, size_t njobs, uint32_t tile_bits,
-	uint32_t low_bits, int use_graph, uint64_t *launches)
+struct ph2hip_batch_plan {
+	FusedPass *desc;      // descriptors, job-major
+	BatchItem *items;     // work items, launch-major
+	size_t ndesc, maxp;
+	size_t *lstart;       // launch p: items [lstart[p], lstart[p+1])
+	uint64_t *ltiles;     // tiles of launch p
+	uint32_t tile_bits;
+	uint64_t namp_total;
+	hipGraph_t graph;
+	hipGraphExec_t exec;
+};
+
+void ph2hip_batch_plan_free(struct ph2hip_batch_plan *pl)
 {
-	if (ensure_init())
-		return -1;
+	if (!pl)
+		return;
+	if (pl->exec)
+		(void)hipGraphExecDestroy(pl->exec);
+	if (pl->graph)
+		(void)hipGraphDestroy(pl->graph);
+	free(pl->desc);
+	free(pl->items);
+	free(pl->lstart);
+	free(pl->ltiles);
+	free(pl);
+}
+
+int ph2hip_batch_plan_build(const struct ph2hip_batch_job *jobs, size_t njobs,
+	uint32_t tile_bits, uint32_t low_bits, uint64_t namp_total,
+	struct ph2hip_batch_plan **out)
+{
+	*out = nullptr;
 	if (tile_bits < PH2HIP_TILE_MIN || tile_bits > PH2HIP_TILE_MAX)
 		return fail("rotate_batched: tile_bits %u", tile_bits);
 	size_t ndesc = 0, maxp = 0;
@@ -1806,128 +1832,151 @@ int ph2hip_rotate_batched(void *amp, uint64_t namp_total,
 		ndesc += J.npass;
 		maxp = J.npass > maxp ? J.npass : maxp;
 	}
-	if (launches)
-		*launches = 0;
-	if (ndesc == 0)
+	struct ph2hip_batch_plan *pl = (struct ph2hip_batch_plan *)calloc(1,
+		sizeof *pl);
+	if (!pl)
+		return fail("rotate_batched: out of host memory");
+	pl->ndesc = ndesc;
+	pl->maxp = maxp;
+	pl->tile_bits = tile_bits;
+	pl->namp_total = namp_total;
+	if (ndesc == 0) {
+		*out = pl;
 		return 0;
-	FusedPass *desc = (FusedPass *)malloc(sizeof(FusedPass) * ndesc);
-	BatchItem *items = (BatchItem *)malloc(sizeof(BatchItem) * ndesc);
-	size_t *first = (size_t *)malloc(sizeof(size_t) * njobs);
-	size_t *lstart = (size_t *)calloc(maxp + 1, sizeof(size_t));
-	uint64_t *ltiles = (uint64_t *)calloc(maxp, sizeof(uint64_t));
-	int rc = 0;
-	if (!desc || !items || !first || !lstart || !ltiles) {
-		rc = fail("rotate_batched: out of host memory");
-		goto out;
 	}
-	// descriptors, job-major
-	{
+	pl->desc = (FusedPass *)malloc(sizeof(FusedPass) * ndesc);
+	pl->items = (BatchItem *)malloc(sizeof(BatchItem) * ndesc);
+	pl->lstart = (size_t *)calloc(maxp + 1, sizeof(size_t));
+	pl->ltiles = (uint64_t *)calloc(maxp, sizeof(uint64_t));
+	size_t *first = (size_t *)malloc(sizeof(size_t) * (njobs ? njobs : 1));
+	int rc = 0;
+	if (!pl->desc || !pl->items || !pl->lstart || !pl->ltiles || !first) {
+		rc = fail("rotate_batched: out of host memory");
+	} else {
+		// descriptors, job-major
 		size_t d = 0;
 		for (size_t j = 0; j < njobs && !rc; j++) {
 			first[j] = d;
 			size_t k0 = 0;
 			for (size_t p = 0; p < jobs[j].npass && !rc; p++) {
-				rc = build_pass(desc[d++], jobs[j].rot + k0,
+				rc = build_pass(pl->desc[d++], jobs[j].rot + k0,
 					jobs[j].pass_len[p], tile_bits, low_bits, k0);
 				k0 += jobs[j].pass_len[p];
 			}
 		}
 	}
-	if (rc)
-		goto out;
-	// work items, launch-major: launch p holds pass p of every job that has it
-	{
+	if (!rc) {
+		// work items, launch-major: launch p holds pass p of every job
+		// that has one
 		size_t it = 0;
 		for (size_t p = 0; p < maxp; p++) {
-			lstart[p] = it;
+			pl->lstart[p] = it;
 			uint64_t t0 = 0;
 			for (size_t j = 0; j < njobs; j++) {
 				if (jobs[j].npass <= p)
 					continue;
-				items[it].amp_off = jobs[j].amp_off;
-				items[it].tile0 = t0;
-				items[it].desc = first[j] + p;
+				pl->items[it].amp_off = jobs[j].amp_off;
+				pl->items[it].tile0 = t0;
+				pl->items[it].desc = first[j] + p;
 				t0 += UINT64_C(1) << (jobs[j].nqb - tile_bits);
 				it++;
 			}
-			ltiles[p] = t0;
+			pl->ltiles[p] = t0;
 		}
-		lstart[maxp] = it;
+		pl->lstart[maxp] = it;
 	}
-	if (dev_reserve(&g_bdesc, &g_bdesc_cap, sizeof(FusedPass) * ndesc) ||
-	    dev_reserve(&g_bitem, &g_bitem_cap, sizeof(BatchItem) * ndesc)) {
-		rc = -1;
-		goto out;
-	}
-	if (hipMemcpyAsync(g_bdesc, desc, sizeof(FusedPass) * ndesc,
-		    hipMemcpyHostToDevice, g_stream) != hipSuccess ||
-	    hipMemcpyAsync(g_bitem, items, sizeof(BatchItem) * ndesc,
-		    hipMemcpyHostToDevice, g_stream) != hipSuccess ||
-	    hipStreamSynchronize(g_stream) != hipSuccess) {
-		rc = fail("rotate_batched: upload");
-		goto out;
-	}
-	{
-		// Profiling brackets every launch with events, which stays out of
-		// graph capture.
-		const bool graph = use_graph && !g_prof;
-		hipGraph_t gr = nullptr;
-		hipGraphExec_t ge = nullptr;
-		if (graph && hipStreamBeginCapture(g_stream,
-			    hipStreamCaptureModeThreadLocal) != hipSuccess) {
-			rc = fail("rotate_batched: begin capture");
-			goto out;
-		}
-		for (size_t p = 0; p < maxp && !rc; p++) {
-			const uint32_t ni = (uint32_t)(lstart[p + 1] - lstart[p]);
-			const BatchItem *it = (const BatchItem *)g_bitem + lstart[p];
-			const FusedPass *dd = (const FusedPass *)g_bdesc;
-			hipEvent_t stop = nullptr;
-			ph2hip_launch_rec meta{};
-			meta.tile_bits = tile_bits;
-			meta.ngroups = ni;
-			if (!graph && prof_begin(3, ni, ltiles[p] << tile_bits,
-				    &stop, &meta)) {
-				rc = -1;
-				break;
-			}
-			double2 *a = (double2 *)amp;
-			switch (tile_bits) {
-			case 9: rc = launch_batch<9>(a, dd, it, ni, ltiles[p]); break;
-			case 10: rc = launch_batch<10>(a, dd, it, ni, ltiles[p]); break;
-			case 11: rc = launch_batch<11>(a, dd, it, ni, ltiles[p]); break;
-			case 12: rc = launch_batch<12>(a, dd, it, ni, ltiles[p]); break;
-			default: rc = launch_batch<13>(a, dd, it, ni, ltiles[p]); break;
-			}
-			if (!rc && !graph && prof_end(stop))
-				rc = -1;
-		}
-		if (graph) {
-			const hipError_t e = hipStreamEndCapture(g_stream, &gr);
-			if (!rc && e != hipSuccess)
-				rc = fail("rotate_batched: end capture: %s",
-					hipGetErrorString(e));
-			if (!rc && hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0) !=
-				hipSuccess)
-				rc = fail("rotate_batched: graph instantiate");
-			if (!rc && hipGraphLaunch(ge, g_stream) != hipSuccess)
-				rc = fail("rotate_batched: graph launch");
-			if (!rc && hipStreamSynchronize(g_stream) != hipSuccess)
-				rc = fail("rotate_batched: graph sync");
-			if (ge)
-				(void)hipGraphExecDestroy(ge);
-			if (gr)
-				(void)hipGraphDestroy(gr);
-		}
-		if (!rc && launches)
-			*launches = maxp;
-	}
-out:
-	free(desc);
-	free(items);
 	free(first);
-	free(lstart);
-	free(ltiles);
+	if (rc) {
+		ph2hip_batch_plan_free(pl);
+		return -1;
+	}
+	*out = pl;
+	return 0;
+}
+
+int ph2hip_batch_plan_launch(void *amp, struct ph2hip_batch_plan *pl,
+	int use_graph, uint64_t *launches)
+{
+	if (ensure_init())
+		return -1;
+	if (launches)
+		*launches = 0;
+	if (pl->ndesc == 0)
+		return 0;
+	const size_t ndesc = pl->ndesc, maxp = pl->maxp;
+	const uint32_t tile_bits = pl->tile_bits;
+	// The upload follows the work already queued on the stream, so the
+	// launches of an earlier plan have read the buffers before they change.
+	if (dev_reserve(&g_bdesc, &g_bdesc_cap, sizeof(FusedPass) * ndesc) ||
+	    dev_reserve(&g_bitem, &g_bitem_cap, sizeof(BatchItem) * ndesc))
+		return -1;
+	if (hipMemcpyAsync(g_bdesc, pl->desc, sizeof(FusedPass) * ndesc,
+		    hipMemcpyHostToDevice, g_stream) != hipSuccess ||
+	    hipMemcpyAsync(g_bitem, pl->items, sizeof(BatchItem) * ndesc,
+		    hipMemcpyHostToDevice, g_stream) != hipSuccess)
+		return fail("rotate_batched: upload");
+	// Profiling brackets every launch with events, which stays out of
+	// graph capture.
+	const bool graph = use_graph && !g_prof;
+	int rc = 0;
+	if (graph && hipStreamBeginCapture(g_stream,
+		    hipStreamCaptureModeThreadLocal) != hipSuccess)
+		return fail("rotate_batched: begin capture");
+	for (size_t p = 0; p < maxp && !rc; p++) {
+		const uint32_t ni = (uint32_t)(pl->lstart[p + 1] - pl->lstart[p]);
+		const BatchItem *it = (const BatchItem *)g_bitem + pl->lstart[p];
+		const FusedPass *dd = (const FusedPass *)g_bdesc;
+		hipEvent_t stop = nullptr;
+		ph2hip_launch_rec meta{};
+		meta.tile_bits = tile_bits;
+		meta.ngroups = ni;
+		if (!graph && prof_begin(3, ni, pl->ltiles[p] << tile_bits, &stop,
+			    &meta)) {
+			rc = -1;
+			break;
+		}
+		double2 *a = (double2 *)amp;
+		const uint64_t nt = pl->ltiles[p];
+		switch (tile_bits) {
+		case 9: rc = launch_batch<9>(a, dd, it, ni, nt); break;
+		case 10: rc = launch_batch<10>(a, dd, it, ni, nt); break;
+		case 11: rc = launch_batch<11>(a, dd, it, ni, nt); break;
+		case 12: rc = launch_batch<12>(a, dd, it, ni, nt); break;
+		default: rc = launch_batch<13>(a, dd, it, ni, nt); break;
+		}
+		if (!rc && !graph && prof_end(stop))
+			rc = -1;
+	}
+	if (graph) {
+		const hipError_t e = hipStreamEndCapture(g_stream, &pl->graph);
+		if (!rc && e != hipSuccess)
+			rc = fail("rotate_batched: end capture: %s",
+				hipGetErrorString(e));
+		if (!rc && hipGraphInstantiate(&pl->exec, pl->graph, nullptr,
+			    nullptr, 0) != hipSuccess)
+			rc = fail("rotate_batched: graph instantiate");
+		if (!rc && hipGraphLaunch(pl->exec, g_stream) != hipSuccess)
+			rc = fail("rotate_batched: graph launch");
+	}
+	if (!rc && launches)
+		*launches = maxp;
+	return rc;
+}
+
+int ph2hip_rotate_batched(void *amp, uint64_t namp_total,
+	const struct ph2hip_batch_job *jobs, size_t njobs, uint32_t tile_bits,
+	uint32_t low_bits, int use_graph, uint64_t *launches)
+{
+	if (ensure_init())
+		return -1;
+	struct ph2hip_batch_plan *pl = nullptr;
+	if (ph2hip_batch_plan_build(jobs, njobs, tile_bits, low_bits, namp_total,
+		    &pl) < 0)
+		return -1;
+	int rc = ph2hip_batch_plan_launch(amp, pl, use_graph, launches);
+	if (hipStreamSynchronize(g_stream) != hipSuccess && !rc)
+		rc = fail("rotate_batched: sync");
+	ph2hip_batch_plan_free(pl);
 	return rc;
 }
 

@@ -352,3 +352,23 @@ def test_batch_qdrift_vs_oracle(oracle, mode, tile):
     if mode == nat.BATCH_FUSED_BATCHED:
         # one launch per pass index instead of one per pass per register
         assert st["launches"] < st["passes"]
+
+
+@pytest.mark.parametrize("tile", [11, 0])
+def test_batch_qdrift_pipelined_rounds(oracle, tile):
+    """All registers at least as wide as the tile: the batched rounds are
+    pipelined (round s+1 drawn and planned on the host while round s runs),
+    with uneven sample counts so registers drop out of later rounds; every
+    register still matches qdrift run on it alone."""
+    regs = _batch_regs([12, 14, 13, 12, 15], depth=150, samples=1)
+    for r, ns in zip(regs, (3, 1, 2, 3, 2)):
+        r["samples"] = ns
+    got, st = nat.batch_qdrift(regs, nat.BATCH_FUSED_BATCHED, tile, 4)
+    print("pipelined", tile, st, flush=True)
+    for r, g in zip(regs, got):
+        want, _ = oracle.qdrift(r["hm"], dict(det_idx=r["det_idx"],
+                                              det_cf=r["det_cf"]),
+                                r["samples"], r["depth"], r["step_size"],
+                                r["seed"], want_state=False)
+        assert np.array_equal(g, want)
+    assert st["launches"] < st["passes"]
