@@ -223,6 +223,13 @@ def plain_rate(nat, nqb, xm, zm, phi):
                 diag_launches=pp["plain_diag"]["launches"])
 
 
+def auto_tile(tile, nqb):
+    """The library's tile width for a requested one (0: automatic)."""
+    if tile:
+        return tile
+    return max(9, 12 if nqb > 18 else nqb - 6)
+
+
 def fused_name(tile, batched=False):
     """Kernel name as rocprofv3 prints it (template arguments TB, D)."""
     d = 4 if tile >= 12 else tile - 8
@@ -307,7 +314,7 @@ def bench_trott_n33(args, nat, costmodel, schedule):
         plain = plain_rate(nat, nqb, xm[sel], zm[sel], phi[sel])
     return dict(
         value=value, elapsed=elapsed, norm2=norm2, plain=plain,
-        roofline=roofline(fused_name(args.tile), fz),
+        roofline=roofline(fused_name(auto_tile(args.tile, nqb)), fz),
         config={"workload": f"trott1_n{nqb}_lex_sweep_blocks", "nqb": nqb,
                 "terms": nterms, "terms_per_step": 1024,
                 "blocks_timed": blocks, "blocks_in_sweep": nblocks,
@@ -366,6 +373,7 @@ def bench_trott_n16(args, nat, costmodel, schedule):
 
     def run(k):
         return nat.trott(1, hm, prep, w["delta"], k, want_state=False)
+    tile = auto_tile(args.tile, hm.nqb)
 
     # profiling adds an event pair per launch, which matters for microsecond
     # launches: time without it, then profile a separate run
@@ -377,7 +385,7 @@ def bench_trott_n16(args, nat, costmodel, schedule):
     nat.synchronize()
     elapsed = time.perf_counter() - t0
     _, prof = _timed_driver(nat, run, args.steps, 0)
-    kern, kp = _dominant(prof, args.tile)
+    kern, kp = _dominant(prof, tile)
     nat.set_default(nat.PLAIN)
     t1 = time.perf_counter()
     run(args.steps)
@@ -413,7 +421,7 @@ def bench_trott2_n26(args, nat, costmodel, schedule):
         return nat.trott(2, hm, prep, w["delta"], k, want_state=False)
 
     elapsed, prof = _timed_driver(nat, run, args.steps, args.warmup)
-    kern, kp = _dominant(prof, args.tile)
+    kern, kp = _dominant(prof, auto_tile(args.tile, hm.nqb))
     nupd = args.steps * 2 * len(hm) * 2.0 ** hm.nqb
     xm, zm, cf = lex(hm, nat)
     sel = spread(len(cf), args.plain_terms)
@@ -445,7 +453,7 @@ def bench_cmpsit_n30(args, nat, costmodel, schedule):
                           want_state=False)
 
     elapsed, prof = _timed_driver(nat, run, args.steps, args.warmup)
-    kern, kp = _dominant(prof, args.tile)
+    kern, kp = _dominant(prof, auto_tile(args.tile, hm.nqb))
     rot_per_step = 2 * (w["length"] + w["depth"])
     nupd = args.steps * rot_per_step * 2.0 ** hm.nqb
     det = np.argsort(-np.abs(hm.cf), kind="stable")[:w["length"]]
@@ -557,7 +565,8 @@ def main(argv=None):
                     help="register width of trott_n33 (smaller for quick runs)")
     ap.add_argument("--regs", type=int, default=512,
                     help="registers of qdrift_batch")
-    ap.add_argument("--tile", type=int, default=12)
+    ap.add_argument("--tile", type=int, default=0,
+                    help="fused tile bits (0: the library's automatic width)")
     ap.add_argument("--low", type=int, default=4)
     ap.add_argument("--plain-terms", type=int, default=24)
     ap.add_argument("--schedule", choices=("auto", "greedy", "learned"),
@@ -609,7 +618,9 @@ def main(argv=None):
     value = r["value"]
     plain = r["plain"]
     cfg = dict(r["config"])
-    cfg.update(schedule=schedule, tile_bits=args.tile, low_bits=args.low)
+    cfg.update(schedule=schedule, low_bits=args.low,
+               tile_bits=auto_tile(args.tile, r["nqb"]) if r["nqb"]
+               else args.tile)
     out = {
         "metric": METRIC,
         "value": value,

@@ -48,8 +48,10 @@ def test_rotate_n13_golden(golden, mode):
     assert po.state_hash(out) == str(g["hash"])
 
 
-@pytest.mark.parametrize("mode", MODES[:2])
+@pytest.mark.parametrize("mode", MODES[:2] + [(nat.FUSED, 0, 4)])
 def test_trott_n16_golden(golden, mode):
+    """The full 537-term, 64-step 16-qubit run, including the automatic
+    tile width (0: tile 10 at 16 qubits)."""
     g = golden("trott_n16.npz")
     w = synth.trott_n16()
     nat.set_default(*mode)
