@@ -97,9 +97,11 @@ int ph2hip_set_fused_form(uint32_t pipe_mask, uint64_t persistent_wgs);
 int ph2hip_set_fused_direct(int on);
 
 
-/* Shape of the fused pass over rot[0..n) without launching it: shape[0] =
- * contiguous low bits L, shape[1] = gathered dimensions g, shape[2] =
- * micro-groups.  -1 if the terms do not fit one tile. */
+/* Shape of the fused pass over rot[0..n) without launching it (shape has 4
+ * entries): shape[0] = contiguous low bits L, shape[1] = gathered
+ * dimensions g, shape[2] = micro-groups, shape[3] = which of the first
+ * (bit 0) and last (bit 1) micro-groups move their cosets straight between
+ * HBM and registers.  -1 if the terms do not fit one tile. */
 int ph2hip_pass_shape(const struct ph2hip_rot *rot, size_t n,
 	uint32_t tile_bits, uint32_t low_bits, uint32_t *shape);
 

@@ -10,7 +10,8 @@ profiling (ph2hip_prof_records).
 Feature vector (must match ph2_learned_features): bits 0..39 of the OR of
 the pass's flip masks, bits 0..39 of the OR of its phase masks, then
 terms/48, micro-groups/48, gathered dims/13, contiguous low bits/13,
-tile bits/13.  Target: ln(ms) - (nqb - 30) ln 2.
+tile bits/13, and whether the first / last micro-group moves its cosets
+straight between HBM and registers.  Target: ln(ms) - (nqb - 30) ln 2.
 """
 
 from __future__ import annotations
@@ -25,7 +26,7 @@ import numpy as np
 from . import _native as nat
 
 NQ = 40
-NIN = 2 * NQ + 5
+NIN = 2 * NQ + 7
 NHID = 64
 PARAMS_PATH = Path(__file__).resolve().parent / "costmodel_params.npz"
 LN2 = math.log(2.0)
@@ -145,6 +146,8 @@ def features_from_records(rec):
     X[:, 2 * NQ + 2] = rec["gdim"] / 13.0
     X[:, 2 * NQ + 3] = rec["low_bits"] / 13.0
     X[:, 2 * NQ + 4] = rec["tile_bits"] / 13.0
+    X[:, 2 * NQ + 5] = rec["direct"] & 1
+    X[:, 2 * NQ + 6] = (rec["direct"] >> 1) & 1
     y = (np.log(rec["ms"]) - (rec["nqb"].astype(np.float64) - 30) * LN2
          ).astype(np.float32)
     return X, y, rec
@@ -156,7 +159,7 @@ def c_features(xm, zm, tile_bits, shape):
     rot = (nat.Rot * max(n, 1))()
     for k in range(n):
         rot[k] = nat.Rot(int(xm[k]), int(zm[k]), 1.0, 0.0)
-    sh = (C.c_uint32 * 3)(*shape)
+    sh = (C.c_uint32 * 4)(*shape)
     f = np.zeros(NIN, np.float32)
     nat.lib().ph2_learned_features(rot, C.c_size_t(n), C.c_uint32(tile_bits),
                                    sh, _p(f, _f32p))

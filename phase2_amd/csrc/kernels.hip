@@ -2083,6 +2083,7 @@ int ph2hip_pass_shape(const struct ph2hip_rot *rot, size_t n,
 	shape[0] = P.L;
 	shape[1] = P.g;
 	shape[2] = P.ngroups;
+	shape[3] = P.dflags;
 	return 0;
 }
 

@@ -53,6 +53,8 @@ void ph2_learned_features(const struct ph2hip_rot *rot, size_t n,
 	f[2 * PH2_LM_NQ + 2] = (float)shape[1] / 13.0f;
 	f[2 * PH2_LM_NQ + 3] = (float)shape[0] / 13.0f;
 	f[2 * PH2_LM_NQ + 4] = (float)tile_bits / 13.0f;
+	f[2 * PH2_LM_NQ + 5] = (float)(shape[3] & 1u);
+	f[2 * PH2_LM_NQ + 6] = (float)(shape[3] >> 1 & 1u);
 }
 
 static uint64_t block_key(const struct ph2hip_rot *rot, size_t n,
@@ -184,7 +186,7 @@ int ph2_learned_schedule(const struct ph2hip_rot *rot, size_t n, uint32_t nqb,
 	/* shapes and features; drop candidates that do not fit */
 	size_t k = 0;
 	for (size_t c = 0; c < m; c++) {
-		uint32_t shape[3];
+		uint32_t shape[4];
 		const struct cand cd = LM.cand[c];
 		if (ph2hip_pass_shape(rot + cd.i, cd.j - cd.i, cd.tb, cd.low,
 			    shape) < 0)
@@ -261,7 +263,7 @@ int ph2_learned_predict(const struct ph2hip_rot *rot, size_t n, uint32_t nqb,
 		return -1;
 	size_t k0 = 0;
 	for (size_t p = 0; p < s->npass; p++) {
-		uint32_t shape[3];
+		uint32_t shape[4];
 		if (k0 + s->len[p] > n ||
 		    ph2hip_pass_shape(rot + k0, s->len[p], s->tile_bits[p],
 			    s->low_bits[p], shape) < 0)

@@ -200,6 +200,9 @@ def main(argv=None):
                     "evaluate (held-out 33-qubit blocks, composite "
                     "integrator at 30 qubits)")
     ap.add_argument("--no-eval", action="store_true")
+    ap.add_argument("--records", default="",
+                    help="train on this saved records.npy instead of "
+                    "collecting new timings")
     ap.add_argument("--cmpsit-steps", type=int, default=1)
     args = ap.parse_args(argv)
     out = Path(args.out)
@@ -217,28 +220,34 @@ def main(argv=None):
     rng = np.random.default_rng(2024)
 
     recs = []
+    train33 = []
+    if args.records:
+        recs.append(np.load(args.records, allow_pickle=False))
+        log(f"records from {args.records}: {len(recs[-1])}")
     # 28 qubits (two Hamiltonians: one holds at most ~80 blocks of distinct
     # four-index strings) and 30 qubits, same generator
-    for nqb, nb, seed in ((28, args.blocks28, 28), (28, args.blocks28, 128),
-                          (30, args.blocks30, 30)):
+    for nqb, nb, seed in () if args.records else (
+            (28, args.blocks28, 28), (28, args.blocks28, 128),
+            (30, args.blocks30, 30)):
         hm = synth.hamiltonian(nqb, 0, seed=seed, decay=True,
                                n_total=1024 * (nb + 4))
         blocks = list(range(2, nb + 2))
         recs.append(collect_width(nqb, hm, blocks, rng))
         log(f"records n={nqb} seed {seed}: {len(recs[-1])}")
     # 33 qubits: blocks of the headline sweep, held-out blocks excluded
-    w = synth.trott_n33()
-    xm33, zm33, phi33 = sorted_terms(w["hm"])
-    nblk = (len(phi33) + 1023) // 1024
-    train33 = [int(b) for b in np.linspace(3, nblk - 2, args.blocks33
-                                           ).astype(int)
-               if b not in EVAL_BLOCKS]
-    reg = nat.Register(33, nat.FUSED, 12, 4)
-    reg.prep_multidet(w["det_idx"], w["det_cf"])
-    recs.append(cm.collect(reg, xm33, zm33, phi33, train33, rng,
-                           log=lambda m: log(f"n=33 {m}")))
-    log(f"records n=33: {len(recs[-1])}")
-    reg.free()
+    if not args.records:
+        w = synth.trott_n33()
+        xm33, zm33, phi33 = sorted_terms(w["hm"])
+        nblk = (len(phi33) + 1023) // 1024
+        train33 = [int(b) for b in np.linspace(3, nblk - 2, args.blocks33
+                                               ).astype(int)
+                   if b not in EVAL_BLOCKS]
+        reg = nat.Register(33, nat.FUSED, 12, 4)
+        reg.prep_multidet(w["det_idx"], w["det_cf"])
+        recs.append(cm.collect(reg, xm33, zm33, phi33, train33, rng,
+                               log=lambda m: log(f"n=33 {m}")))
+        log(f"records n=33: {len(recs[-1])}")
+        reg.free()
     for f in sorted(glob.glob(args.extra_records)) if args.extra_records else []:
         r = np.load(f, allow_pickle=False)
         recs.append(r[r["kind"] == 2])

@@ -16,10 +16,10 @@ def test_features_match_c():
     rng = np.random.default_rng(0)
     xm = rng.integers(0, 1 << 33, size=7, dtype=np.uint64)
     zm = rng.integers(0, 1 << 33, size=7, dtype=np.uint64)
-    f = cm.c_features(xm, zm, 12, (6, 5, 3))
+    f = cm.c_features(xm, zm, 12, (6, 5, 3, 2))
     rec = np.zeros(1, dtype=nat.REC_DTYPE)
     rec["kind"], rec["nqb"], rec["tile_bits"], rec["low_bits"] = 2, 33, 12, 6
-    rec["gdim"], rec["nterms"], rec["ngroups"] = 5, 7, 3
+    rec["gdim"], rec["nterms"], rec["ngroups"], rec["direct"] = 5, 7, 3, 2
     rec["xunion"] = np.bitwise_or.reduce(xm)
     rec["zunion"] = np.bitwise_or.reduce(zm)
     rec["ms"] = 1.0
