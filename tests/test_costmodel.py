@@ -72,11 +72,19 @@ def test_mlp_forward_grad_vs_oracle():
                                            cm.NHID)
     assert abs(loss - loss_ref) <= 1e-4 * max(1, loss_ref)
     assert np.max(np.abs(g - g_ref)) <= 1e-4 * max(1, np.max(np.abs(g_ref)))
-    # Adam steps on a fixed batch reduce the loss
-    l0 = m.train_step(X, t, 1e-3)
-    for _ in range(50):
-        l1 = m.train_step(X, t, 1e-3)
-    assert l1 < 0.5 * l0
+    # First Adam step (bias-corrected moments m = g, v = g^2): every
+    # parameter moves by -lr * g / (|g| + eps), eps = 1e-8
+    lr = 1e-3
+    l0 = m.train_step(X, t, lr)
+    assert abs(l0 - loss_ref) <= 1e-4 * max(1, loss_ref)
+    p1 = m.get_params().astype(np.float64)
+    g64 = g.astype(np.float64)
+    want = p - lr * g64 / (np.abs(g64) + 1e-8)
+    assert np.max(np.abs(p1 - want)) <= 1e-6
+    # and further steps on the fixed batch reduce the loss
+    for _ in range(100):
+        l1 = m.train_step(X, t, lr)
+    assert l1 < 0.8 * l0
     m.close()
 
 
