@@ -288,6 +288,7 @@ def bench_trott_n33(args, nat, costmodel, schedule):
 
     nat.prof_enable(True)
     nat.prof_read(reset=True)
+    nat.prof_records(reset=True)
     nupd = 0.0
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -299,11 +300,18 @@ def bench_trott_n33(args, nat, costmodel, schedule):
     nat.synchronize()
     elapsed = time.perf_counter() - t0
     prof = nat.prof_read(reset=True)
+    rec = nat.prof_records(reset=True)
     nat.prof_enable(False)
     norm2 = reg.norm2()
     reg.free()
     value = nupd / elapsed
     fz = prof["fused"]
+    rec = rec[rec["kind"] == 2]
+    tiles = {int(t): int(c) for t, c in
+             zip(*np.unique(rec["tile_bits"], return_counts=True))}
+    # the dominant kernel: the tile width with most launch time
+    tb_main = max(tiles, key=lambda t: rec["ms"][rec["tile_bits"] == t].sum()
+                  ) if tiles else auto_tile(args.tile, nqb)
     plain = None
     if not args.no_plain:
         # terms spread evenly through each timed block (the per-pass cost of
@@ -314,12 +322,13 @@ def bench_trott_n33(args, nat, costmodel, schedule):
         plain = plain_rate(nat, nqb, xm[sel], zm[sel], phi[sel])
     return dict(
         value=value, elapsed=elapsed, norm2=norm2, plain=plain,
-        roofline=roofline(fused_name(auto_tile(args.tile, nqb)), fz),
+        roofline=roofline(fused_name(tb_main), fz),
         config={"workload": f"trott1_n{nqb}_lex_sweep_blocks", "nqb": nqb,
                 "terms": nterms, "terms_per_step": 1024,
                 "blocks_timed": blocks, "blocks_in_sweep": nblocks,
                 "delta": w["delta"]},
         extra={"terms_per_pass": fz["terms"] / max(1, fz["launches"]),
+               "launches_per_tile_width": tiles,
                "first_visit_seconds": sched_s if schedule == "learned"
                else None},
         cpu=lambda ranks: cpu_n33_proxy(args, nat, ranks),
