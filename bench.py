@@ -236,6 +236,33 @@ def fused_name(tile, batched=False):
     return f"k_rot_fused{'_batch' if batched else ''}<{tile}, {d}>"
 
 
+REC_FUSED = 2   # launch-record kind of a single-register fused pass
+
+
+def fused_by_tile(rec):
+    """Per tile width, the fused-pass launches of a record array: launches,
+    summed ms, amplitudes swept (2^nqb each) and terms."""
+    rec = rec[rec["kind"] == REC_FUSED]
+    out = {}
+    for t in np.unique(rec["tile_bits"]):
+        r = rec[rec["tile_bits"] == t]
+        out[int(t)] = {"launches": len(r), "ms": float(r["ms"].sum()),
+                       "amps": float(np.sum(2.0 ** r["nqb"].astype(float))),
+                       "terms": int(r["nterms"].sum())}
+    return out
+
+
+def dominant_fused(rec, prof):
+    """(kernel name, stats) of the fused kernel template with the most
+    launch time, and the launch count per tile width.  Falls back to the
+    aggregate when the records are empty."""
+    per = fused_by_tile(rec)
+    if not per:
+        return None, prof["fused"], {}
+    tb = max(per, key=lambda t: per[t]["ms"])
+    return fused_name(tb), per[tb], {t: v["launches"] for t, v in per.items()}
+
+
 def spread(n, k):
     return np.unique(np.linspace(0, n - 1, min(n, k)).astype(int))
 
@@ -306,12 +333,7 @@ def bench_trott_n33(args, nat, costmodel, schedule):
     reg.free()
     value = nupd / elapsed
     fz = prof["fused"]
-    rec = rec[rec["kind"] == 2]
-    tiles = {int(t): int(c) for t, c in
-             zip(*np.unique(rec["tile_bits"], return_counts=True))}
-    # the dominant kernel: the tile width with most launch time
-    tb_main = max(tiles, key=lambda t: rec["ms"][rec["tile_bits"] == t].sum()
-                  ) if tiles else auto_tile(args.tile, nqb)
+    kern, kp, tiles = dominant_fused(rec, prof)
     plain = None
     if not args.no_plain:
         # terms spread evenly through each timed block (the per-pass cost of
@@ -322,7 +344,7 @@ def bench_trott_n33(args, nat, costmodel, schedule):
         plain = plain_rate(nat, nqb, xm[sel], zm[sel], phi[sel])
     return dict(
         value=value, elapsed=elapsed, norm2=norm2, plain=plain,
-        roofline=roofline(fused_name(tb_main), fz),
+        roofline=roofline(kern, kp),
         config={"workload": f"trott1_n{nqb}_lex_sweep_blocks", "nqb": nqb,
                 "terms": nterms, "terms_per_step": 1024,
                 "blocks_timed": blocks, "blocks_in_sweep": nblocks,
@@ -357,20 +379,26 @@ def _timed_driver(nat, run, steps, warmup):
     nat.synchronize()
     nat.prof_enable(True)
     nat.prof_read(reset=True)
+    nat.prof_records(reset=True)
     t0 = time.perf_counter()
     run(steps)
     nat.synchronize()
     elapsed = time.perf_counter() - t0
     prof = nat.prof_read(reset=True)
+    prof["records"] = nat.prof_records(reset=True)
     nat.prof_enable(False)
     return elapsed, prof
 
 
-def _dominant(prof, tile):
-    kinds = {"fused": fused_name(tile), "plain_pair": "k_rot_pair",
-             "plain_diag": "k_rot_diag", "batched": fused_name(tile, True)}
-    k = max(kinds, key=lambda n: prof[n]["ms"])
-    return kinds[k], prof[k]
+def _dominant(prof):
+    """The kernel with most launch time: a fused template (per tile width,
+    from the launch records) or the plain pair/diagonal kernel."""
+    kern, kp, tiles = dominant_fused(prof["records"], prof)
+    best = (kern, kp) if kern else (None, {"ms": -1.0})
+    for k, name in (("plain_pair", "k_rot_pair"), ("plain_diag", "k_rot_diag")):
+        if prof[k]["ms"] > best[1]["ms"]:
+            best = (name, prof[k])
+    return best[0], best[1], tiles
 
 
 def bench_trott_n16(args, nat, costmodel, schedule):
@@ -382,7 +410,6 @@ def bench_trott_n16(args, nat, costmodel, schedule):
 
     def run(k):
         return nat.trott(1, hm, prep, w["delta"], k, want_state=False)
-    tile = auto_tile(args.tile, hm.nqb)
 
     # profiling adds an event pair per launch, which matters for microsecond
     # launches: time without it, then profile a separate run
@@ -394,7 +421,7 @@ def bench_trott_n16(args, nat, costmodel, schedule):
     nat.synchronize()
     elapsed = time.perf_counter() - t0
     _, prof = _timed_driver(nat, run, args.steps, 0)
-    kern, kp = _dominant(prof, tile)
+    kern, kp, tiles = _dominant(prof)
     nat.set_default(nat.PLAIN)
     t1 = time.perf_counter()
     run(args.steps)
@@ -409,7 +436,8 @@ def bench_trott_n16(args, nat, costmodel, schedule):
         roofline=roofline(kern, kp),
         config={"workload": "trott1_n16_full", "nqb": 16, "terms": len(hm),
                 "steps": args.steps, "delta": w["delta"]},
-        extra={"overlap_last": [float(out[-1].real), float(out[-1].imag)]},
+        extra={"overlap_last": [float(out[-1].real), float(out[-1].imag)],
+               "launches_per_tile_width": tiles},
         cpu=lambda ranks: cpu_circ(
             1, synth.trott_n16()["hm"], w["det_idx"], w["det_cf"], ranks,
             f"trott n=16, {args.steps} steps of {len(hm)} terms",
@@ -430,7 +458,7 @@ def bench_trott2_n26(args, nat, costmodel, schedule):
         return nat.trott(2, hm, prep, w["delta"], k, want_state=False)
 
     elapsed, prof = _timed_driver(nat, run, args.steps, args.warmup)
-    kern, kp = _dominant(prof, auto_tile(args.tile, hm.nqb))
+    kern, kp, tiles = _dominant(prof)
     nupd = args.steps * 2 * len(hm) * 2.0 ** hm.nqb
     xm, zm, cf = lex(hm, nat)
     sel = spread(len(cf), args.plain_terms)
@@ -442,7 +470,9 @@ def bench_trott2_n26(args, nat, costmodel, schedule):
         config={"workload": "trott2_n26_coeff_matrix", "nqb": 26,
                 "terms": len(hm), "steps": args.steps, "delta": w["delta"],
                 "determinants": 230230},
-        extra={"terms_per_pass": kp["terms"] / max(1, kp["launches"])},
+        extra={"terms_per_pass": prof["fused"]["terms"]
+               / max(1, prof["fused"]["launches"]),
+               "launches_per_tile_width": tiles},
         cpu=lambda ranks: cpu_rotations(
             26, xm[:256], zm[:256], 0.5 * w["delta"] * cf[:256], ranks,
             "first 256 rotations of a trott2 forward half-sweep"),
@@ -462,7 +492,7 @@ def bench_cmpsit_n30(args, nat, costmodel, schedule):
                           want_state=False)
 
     elapsed, prof = _timed_driver(nat, run, args.steps, args.warmup)
-    kern, kp = _dominant(prof, auto_tile(args.tile, hm.nqb))
+    kern, kp, tiles = _dominant(prof)
     rot_per_step = 2 * (w["length"] + w["depth"])
     nupd = args.steps * rot_per_step * 2.0 ** hm.nqb
     det = np.argsort(-np.abs(hm.cf), kind="stable")[:w["length"]]
@@ -480,7 +510,9 @@ def bench_cmpsit_n30(args, nat, costmodel, schedule):
                 "length": w["length"], "depth": w["depth"],
                 "steps": args.steps, "rotations_per_step": rot_per_step,
                 "angle_det": w["angle_det"], "angle_rand": w["angle_rand"]},
-        extra={"terms_per_pass": kp["terms"] / max(1, kp["launches"])},
+        extra={"terms_per_pass": prof["fused"]["terms"]
+               / max(1, prof["fused"]["launches"]),
+               "launches_per_tile_width": tiles},
         cpu=lambda ranks: cpu_rotations(
             30, dx[:16], dz[:16], phi[:16], ranks,
             "first 16 deterministic rotations of a cmpsit half-step"),
@@ -609,7 +641,7 @@ def main(argv=None):
         schedule = "learned" if costmodel.PARAMS_PATH.is_file() else "greedy"
     if schedule == "learned":
         params, _ = costmodel.load()
-        costmodel.install(params, min_qb=28)
+        costmodel.install(params, min_qb=30)
 
     r = BENCH[args.workload](args, nat, costmodel, schedule)
     if schedule == "learned":

@@ -325,6 +325,83 @@ __device__ __forceinline__ double flip_sign(double v, uint32_t f)
 		__double2loint(v));
 }
 
+// One pair update in place: (x, y) of one coset pair, c uniform, si the
+// per-thread signed sine.  ODD:  x <- c x + si y,  y <- c y - si x;
+// even:  x <- c x - si (i y),  y <- c y - si (i x)  per component as in
+// rot_pair.  Each result is round(round(c u) +- round(si v)), the same
+// roundings as add(mul(c, u), mul(+-si, v)) (negation is exact).  Written
+// as one block with the four values read and written in place: as plain
+// code the compiler gives each term's results new registers and then
+// copies the whole coset back once per term at the loop back-edge.
+template <bool ODD>
+__device__ __forceinline__ void pair_inplace(double &xr, double &xi,
+	double &yr, double &yi, double c, double si)
+{
+	double t0, t1, t2, t3, t4, t5, t6, t7;
+	if (ODD)
+		__asm__(
+			"v_mul_f64 %[t0], %[c], %[xr]\n\t"
+			"v_mul_f64 %[t1], %[si], %[yr]\n\t"
+			"v_mul_f64 %[t2], %[c], %[yr]\n\t"
+			"v_mul_f64 %[t3], %[si], %[xr]\n\t"
+			"v_mul_f64 %[t4], %[c], %[xi]\n\t"
+			"v_mul_f64 %[t5], %[si], %[yi]\n\t"
+			"v_mul_f64 %[t6], %[c], %[yi]\n\t"
+			"v_mul_f64 %[t7], %[si], %[xi]\n\t"
+			"v_add_f64 %[xr], %[t0], %[t1]\n\t"
+			"v_add_f64 %[yr], %[t2], -%[t3]\n\t"
+			"v_add_f64 %[xi], %[t4], %[t5]\n\t"
+			"v_add_f64 %[yi], %[t6], -%[t7]"
+			: [xr] "+v"(xr), [xi] "+v"(xi), [yr] "+v"(yr),
+			  [yi] "+v"(yi), [t0] "=&v"(t0), [t1] "=&v"(t1),
+			  [t2] "=&v"(t2), [t3] "=&v"(t3), [t4] "=&v"(t4),
+			  [t5] "=&v"(t5), [t6] "=&v"(t6), [t7] "=&v"(t7)
+			: [c] "s"(c), [si] "v"(si));
+	else
+		__asm__(
+			"v_mul_f64 %[t0], %[c], %[xr]\n\t"
+			"v_mul_f64 %[t1], %[si], %[yi]\n\t"
+			"v_mul_f64 %[t2], %[c], %[xi]\n\t"
+			"v_mul_f64 %[t3], %[si], %[yr]\n\t"
+			"v_mul_f64 %[t4], %[c], %[yr]\n\t"
+			"v_mul_f64 %[t5], %[si], %[xi]\n\t"
+			"v_mul_f64 %[t6], %[c], %[yi]\n\t"
+			"v_mul_f64 %[t7], %[si], %[xr]\n\t"
+			"v_add_f64 %[xr], %[t0], -%[t1]\n\t"
+			"v_add_f64 %[xi], %[t2], %[t3]\n\t"
+			"v_add_f64 %[yr], %[t4], -%[t5]\n\t"
+			"v_add_f64 %[yi], %[t6], %[t7]"
+			: [xr] "+v"(xr), [xi] "+v"(xi), [yr] "+v"(yr),
+			  [yi] "+v"(yi), [t0] "=&v"(t0), [t1] "=&v"(t1),
+			  [t2] "=&v"(t2), [t3] "=&v"(t3), [t4] "=&v"(t4),
+			  [t5] "=&v"(t5), [t6] "=&v"(t6), [t7] "=&v"(t7)
+			: [c] "s"(c), [si] "v"(si));
+}
+
+// Diagonal update in place: a <- c a + sj (i a) = (c re - sj im,
+// c im + sj re), the roundings of rot_diag.
+__device__ __forceinline__ void diag_inplace(double &re, double &im,
+	double c, double sj)
+{
+	double t0, t1, t2, t3;
+	__asm__(
+		"v_mul_f64 %[t0], %[c], %[re]\n\t"
+		"v_mul_f64 %[t1], %[sj], %[im]\n\t"
+		"v_mul_f64 %[t2], %[c], %[im]\n\t"
+		"v_mul_f64 %[t3], %[sj], %[re]\n\t"
+		"v_add_f64 %[re], %[t0], -%[t1]\n\t"
+		"v_add_f64 %[im], %[t2], %[t3]"
+		: [re] "+v"(re), [im] "+v"(im), [t0] "=&v"(t0),
+		  [t1] "=&v"(t1), [t2] "=&v"(t2), [t3] "=&v"(t3)
+		: [c] "s"(c), [sj] "v"(sj));
+}
+
+// +-1.0 with the sign bit f (an exact multiplier)
+__device__ __forceinline__ double unit_sign(uint32_t f)
+{
+	return __hiloint2double(0x3ff00000 | (int)(f << 31), 0);
+}
+
 // One term with compile-time flip pattern XT on a coset of 2^D registers.
 // r4 = y + 2*par (mod 4) with par = bpt ^ parity(w & ztp).  Odd y: the
 // partner enters as itself and s_j = -s_i; even y: as i*partner and
@@ -335,42 +412,28 @@ __device__ __forceinline__ void coset_term(double (&re)[1 << D],
 	double (&im)[1 << D], uint32_t ztp, double sb, double c)
 {
 	constexpr int H = 31 - __builtin_clz(XT);
-	// sb's words, kept opaque so that the compiler materialises them once
-	// per term instead of re-deriving the per-thread sign in every pair;
-	// per pair only the uniform sign bit (scalar unit) is XORed in.
-	int sbh = __double2hiint(sb), sbl = __double2loint(sb);
-	__asm__("" : "+v"(sbh));
 #pragma unroll
 	for (int q = 0; q < (1 << (D - 1)); q++) {
 		const int w = ((q >> H) << (H + 1)) | (q & ((1 << H) - 1));
-		const double si = __hiloint2double(
-			sbh ^ (int)((__popc(w & ztp) & 1u) << 31), sbl);
-		double &xr = re[w], &xi = im[w];
-		double &yr = re[w ^ XT], &yi = im[w ^ XT];
-		double nxr, nxi, nyr, nyi;
-		if (ODD) {
-			nxr = add(mul(c, xr), mul(si, yr));
-			nxi = add(mul(c, xi), mul(si, yi));
-			nyr = add(mul(c, yr), mul(-si, xr));
-			nyi = add(mul(c, yi), mul(-si, xi));
-		} else {
-			nxr = add(mul(c, xr), mul(si, -yi));
-			nxi = add(mul(c, xi), mul(si, yr));
-			nyr = add(mul(c, yr), mul(si, -xi));
-			nyi = add(mul(c, yi), mul(si, xr));
-		}
-		xr = nxr;
-		xi = nxi;
-		yr = nyr;
-		yi = nyi;
+		// per pair only the uniform sign (scalar unit) changes: one
+		// exact multiply by +-1.0 held in scalar registers
+		const double si = mul(sb,
+			unit_sign(__popc(w & ztp) & 1u));
+		pair_inplace<ODD>(re[w], im[w], re[w ^ XT], im[w ^ XT], c, si);
 	}
 }
 
+// xtp is in 1 .. 2^D - 1 (built so on the host); the last pattern takes
+// every value not matched before it, so that no path leaves the coset
+// unchanged: such a path makes the compiler copy all 2^(D+1) coset
+// registers once per term at the loop back-edge.
 template <int D, bool ODD, int XT = 1>
 __device__ __forceinline__ void coset_dispatch(double (&re)[1 << D],
 	double (&im)[1 << D], uint32_t xtp, uint32_t ztp, double sb, double c)
 {
-	if constexpr (XT < (1 << D)) {
+	if constexpr (XT == (1 << D) - 1) {
+		coset_term<D, XT, ODD>(re, im, ztp, sb, c);
+	} else {
 		if (xtp == XT) {
 			coset_term<D, XT, ODD>(re, im, ztp, sb, c);
 			return;
@@ -467,16 +530,9 @@ __device__ __forceinline__ void tile_groups(v2d *__restrict__ a, v2d *tile,
 					// diagonal: a <- c a + s_j (i a), s_j = +-s
 					const double sb = flip_sign(ts, bpt);
 #pragma unroll
-					for (int w = 0; w < (1 << D); w++) {
-						const double sj = flip_sign(sb,
-							__popc(w & ztp) & 1u);
-						const double nr = add(mul(c, re[w]),
-							mul(sj, -im[w]));
-						const double ni = add(mul(c, im[w]),
-							mul(sj, re[w]));
-						re[w] = nr;
-						im[w] = ni;
-					}
+					for (int w = 0; w < (1 << D); w++)
+						diag_inplace(re[w], im[w], c, mul(sb,
+							unit_sign(__popc(w & ztp) & 1u)));
 				} else {
 					const double sb = flip_sign(ts,
 						bpt ^ ((ty >> 1) & 1u));

@@ -1,0 +1,69 @@
+"""Host-side pieces of bench.py: kernel naming, the choice of the dominant
+fused template from per-launch records, the roofline arithmetic, and the
+one-GPU guard.  No GPU call."""
+
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+sys.path.insert(0, str(ROOT))
+import bench  # noqa: E402
+from phase2_amd import _native  # noqa: E402
+
+
+def test_fused_name():
+    assert bench.fused_name(12) == "k_rot_fused<12, 4>"
+    assert bench.fused_name(11) == "k_rot_fused<11, 3>"
+    assert bench.fused_name(10, True) == "k_rot_fused_batch<10, 2>"
+
+
+def _recs(rows):
+    r = np.zeros(len(rows), dtype=_native.REC_DTYPE)
+    for i, (kind, nqb, tb, nt, ms) in enumerate(rows):
+        r[i]["kind"], r[i]["nqb"], r[i]["tile_bits"] = kind, nqb, tb
+        r[i]["nterms"], r[i]["ms"] = nt, ms
+    return r
+
+
+def test_dominant_fused_by_time_not_count():
+    # three short tile-11 launches, two long tile-12 launches, one record of
+    # another kind that must be ignored
+    rec = _recs([(2, 33, 11, 10, 10.0)] * 3 + [(2, 33, 12, 20, 40.0)] * 2
+                + [(0, 33, 0, 1, 500.0)])
+    name, st, tiles = bench.dominant_fused(rec, {"fused": None})
+    assert name == "k_rot_fused<12, 4>"
+    assert tiles == {11: 3, 12: 2}
+    assert st["launches"] == 2 and st["ms"] == 80.0
+    assert st["amps"] == 2 * 2.0 ** 33 and st["terms"] == 40
+    rf = bench.roofline(name, st)
+    # 32 B per amplitude, 40 ms per launch
+    assert rf["algorithmic_bytes_per_launch"] == 32 * 2 ** 33
+    assert rf["achieved"] == pytest.approx(32 * 2 ** 33 / 0.040 / 1e9)
+    assert rf["frac"] == pytest.approx(rf["achieved"] / 8000.0)
+
+
+def test_dominant_falls_back_without_records():
+    agg = {"launches": 4, "ms": 8.0, "amps": 4 * 2.0 ** 20, "terms": 12}
+    name, st, tiles = bench.dominant_fused(_recs([]), {"fused": agg})
+    assert name is None and st is agg and tiles == {}
+
+
+def test_dominant_prefers_plain_when_it_dominates():
+    prof = {"records": _recs([(2, 20, 12, 5, 1.0)]),
+            "plain_pair": {"launches": 9, "ms": 5.0, "amps": 9 * 2.0 ** 20,
+                           "terms": 9},
+            "plain_diag": {"launches": 0, "ms": 0.0, "amps": 0.0,
+                           "terms": 0}}
+    name, st, _ = bench._dominant(prof)
+    assert name == "k_rot_pair" and st["ms"] == 5.0
+
+
+def test_more_than_one_gpu_exits_with_message():
+    p = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode != 0
+    assert "one GPU" in (p.stdout + p.stderr)
