@@ -327,12 +327,14 @@ __device__ __forceinline__ double flip_sign(double v, uint32_t f)
 
 // One pair update in place: (x, y) of one coset pair, c uniform, si the
 // per-thread signed sine.  ODD:  x <- c x + si y,  y <- c y - si x;
-// even:  x <- c x - si (i y),  y <- c y - si (i x)  per component as in
+// even:  x <- c x + si (i y),  y <- c y + si (i x), per component as in
 // rot_pair.  Each result is round(round(c u) +- round(si v)), the same
 // roundings as add(mul(c, u), mul(+-si, v)) (negation is exact).  Written
 // as one block with the four values read and written in place: as plain
 // code the compiler gives each term's results new registers and then
-// copies the whole coset back once per term at the loop back-edge.
+// copies the whole coset back once per term at the loop back-edge.  The
+// cosine products come first, so si (computed just before) is not waited
+// on by the first four multiplies.
 template <bool ODD>
 __device__ __forceinline__ void pair_inplace(double &xr, double &xi,
 	double &yr, double &yi, double c, double si)
@@ -341,12 +343,12 @@ __device__ __forceinline__ void pair_inplace(double &xr, double &xi,
 	if (ODD)
 		__asm__(
 			"v_mul_f64 %[t0], %[c], %[xr]\n\t"
-			"v_mul_f64 %[t1], %[si], %[yr]\n\t"
 			"v_mul_f64 %[t2], %[c], %[yr]\n\t"
-			"v_mul_f64 %[t3], %[si], %[xr]\n\t"
 			"v_mul_f64 %[t4], %[c], %[xi]\n\t"
-			"v_mul_f64 %[t5], %[si], %[yi]\n\t"
 			"v_mul_f64 %[t6], %[c], %[yi]\n\t"
+			"v_mul_f64 %[t1], %[si], %[yr]\n\t"
+			"v_mul_f64 %[t3], %[si], %[xr]\n\t"
+			"v_mul_f64 %[t5], %[si], %[yi]\n\t"
 			"v_mul_f64 %[t7], %[si], %[xi]\n\t"
 			"v_add_f64 %[xr], %[t0], %[t1]\n\t"
 			"v_add_f64 %[yr], %[t2], -%[t3]\n\t"
@@ -360,12 +362,12 @@ __device__ __forceinline__ void pair_inplace(double &xr, double &xi,
 	else
 		__asm__(
 			"v_mul_f64 %[t0], %[c], %[xr]\n\t"
-			"v_mul_f64 %[t1], %[si], %[yi]\n\t"
 			"v_mul_f64 %[t2], %[c], %[xi]\n\t"
-			"v_mul_f64 %[t3], %[si], %[yr]\n\t"
 			"v_mul_f64 %[t4], %[c], %[yr]\n\t"
-			"v_mul_f64 %[t5], %[si], %[xi]\n\t"
 			"v_mul_f64 %[t6], %[c], %[yi]\n\t"
+			"v_mul_f64 %[t1], %[si], %[yi]\n\t"
+			"v_mul_f64 %[t3], %[si], %[yr]\n\t"
+			"v_mul_f64 %[t5], %[si], %[xi]\n\t"
 			"v_mul_f64 %[t7], %[si], %[xr]\n\t"
 			"v_add_f64 %[xr], %[t0], -%[t1]\n\t"
 			"v_add_f64 %[xi], %[t2], %[t3]\n\t"
@@ -386,8 +388,8 @@ __device__ __forceinline__ void diag_inplace(double &re, double &im,
 	double t0, t1, t2, t3;
 	__asm__(
 		"v_mul_f64 %[t0], %[c], %[re]\n\t"
-		"v_mul_f64 %[t1], %[sj], %[im]\n\t"
 		"v_mul_f64 %[t2], %[c], %[im]\n\t"
+		"v_mul_f64 %[t1], %[sj], %[im]\n\t"
 		"v_mul_f64 %[t3], %[sj], %[re]\n\t"
 		"v_add_f64 %[re], %[t0], -%[t1]\n\t"
 		"v_add_f64 %[im], %[t2], %[t3]"
@@ -473,6 +475,12 @@ __device__ __forceinline__ void tile_groups(v2d *__restrict__ a, v2d *tile,
 		const uint32_t fcw = P.grp[gi].fcw;
 		const uint32_t gu[4] = { uw0 & 0xffffu, uw0 >> 16, uw1 & 0xffffu,
 			uw1 >> 16 };
+		// byte offsets of the basis vectors' LDS slots (the slot map is
+		// linear, so an element's offset is the XOR of these)
+		uint32_t gs[4];
+#pragma unroll
+		for (int i = 0; i < 4; i++)
+			gs[i] = lds_slot(gu[i]) << 4;
 		const uint32_t jw[3] = { jw0, jw1, jw2 };
 		const bool gload = gi == 0 && (dflags & 1u);
 		const bool gstore = gi + 1 == ngroups && (dflags & 2u);
@@ -485,7 +493,12 @@ __device__ __forceinline__ void tile_groups(v2d *__restrict__ a, v2d *tile,
 			for (int k = 0; k < TB - D; k++)
 				bu |= (j >> k & 1u) <<
 					((jw[k >> 2] >> (8 * (k & 3))) & 0xffu);
-			const uint32_t sbu = lds_slot(bu);
+			// byte offset of the coset base's slot: the per-element
+			// offsets (uniform) are XORed in pre-shifted, one VALU
+			// operation per LDS access
+			uint32_t sbu16 = lds_slot(bu) << 4;
+			__asm__("" : "+v"(sbu16));   // keeps the shift out of
+			                             // the per-access XOR
 			// HBM index of coordinate bu (row part ^ column part)
 			uint64_t abu = 0;
 			if (gload || gstore) {
@@ -505,22 +518,30 @@ __device__ __forceinline__ void tile_groups(v2d *__restrict__ a, v2d *tile,
 			} else {
 #pragma unroll
 				for (int w = 0; w < (1 << D); w++) {
-					uint32_t cw = 0;
+					uint32_t cw16 = 0;
 #pragma unroll
 					for (int i = 0; i < D; i++)
 						if (w >> i & 1)
-							cw ^= gu[i];
-					const v2d v = tile[sbu ^ lds_slot(cw)];
+							cw16 ^= gs[i];
+					const v2d v = *reinterpret_cast<const v2d *>(
+						reinterpret_cast<const char *>(tile) +
+						(sbu16 ^ cw16));
 					re[w] = v.x;
 					im[w] = v.y;
 				}
 			}
 			const uint32_t kfirst = fcw & 0xffu;
 			const uint32_t kend = kfirst + ((fcw >> 8) & 0xffu);
+			// term k + 1 is loaded (scalar loads) while term k runs
+			uint64_t tz = P.t[kfirst].z;
+			double c = P.t[kfirst].c, ts = P.t[kfirst].s;
+			uint32_t tzt = P.t[kfirst].zt, meta = P.t[kfirst].meta;
 			for (uint32_t k = kfirst; k < kend; k++) {
-				const uint64_t tz = P.t[k].z;
-				const double c = P.t[k].c, ts = P.t[k].s;
-				const uint32_t tzt = P.t[k].zt, meta = P.t[k].meta;
+				const uint32_t kn = k + 1 < kend ? k + 1 : k;
+				const uint64_t tz_n = P.t[kn].z;
+				const double c_n = P.t[kn].c, ts_n = P.t[kn].s;
+				const uint32_t tzt_n = P.t[kn].zt,
+					meta_n = P.t[kn].meta;
 				const uint32_t xtp = meta & 0xffu;
 				const uint32_t ztp = (meta >> 8) & 0xffu;
 				const uint32_t ty = (meta >> 16) & 0xffu;
@@ -543,6 +564,11 @@ __device__ __forceinline__ void tile_groups(v2d *__restrict__ a, v2d *tile,
 						coset_dispatch<D, false>(re, im, xtp,
 							ztp, sb, c);
 				}
+				tz = tz_n;
+				c = c_n;
+				ts = ts_n;
+				tzt = tzt_n;
+				meta = meta_n;
 			}
 			if (gstore) {
 #pragma unroll
@@ -555,15 +581,17 @@ __device__ __forceinline__ void tile_groups(v2d *__restrict__ a, v2d *tile,
 			} else {
 #pragma unroll
 				for (int w = 0; w < (1 << D); w++) {
-					uint32_t cw = 0;
+					uint32_t cw16 = 0;
 #pragma unroll
 					for (int i = 0; i < D; i++)
 						if (w >> i & 1)
-							cw ^= gu[i];
+							cw16 ^= gs[i];
 					v2d v;
 					v.x = re[w];
 					v.y = im[w];
-					tile[sbu ^ lds_slot(cw)] = v;
+					*reinterpret_cast<v2d *>(
+						reinterpret_cast<char *>(tile) +
+						(sbu16 ^ cw16)) = v;
 				}
 			}
 		}
