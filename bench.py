@@ -230,37 +230,47 @@ def auto_tile(tile, nqb):
     return max(9, 12 if nqb > 18 else nqb - 6)
 
 
-def fused_name(tile, batched=False):
-    """Kernel name as rocprofv3 prints it (template arguments TB, D)."""
+def fused_name(tile, batched=False, half=False):
+    """Kernel name as rocprofv3 prints it (template arguments TB, D, and
+    for the batched kernel the half-tile form)."""
     d = 4 if tile >= 12 else tile - 8
-    return f"k_rot_fused{'_batch' if batched else ''}<{tile}, {d}>"
+    if batched:
+        return f"k_rot_fused_batch<{tile}, {d}, {'true' if half else 'false'}>"
+    return f"k_rot_fused{'_half' if half else ''}<{tile}, {d}>"
 
 
 REC_FUSED = 2   # launch-record kind of a single-register fused pass
+REC_BATCH = 3   # launch-record kind of a batched fused pass
+REC_HALF = 4    # launch-record flag: half-tile form
 
 
-def fused_by_tile(rec):
-    """Per tile width, the fused-pass launches of a record array: launches,
-    summed ms, amplitudes swept (2^nqb each) and terms."""
-    rec = rec[rec["kind"] == REC_FUSED]
+def fused_by_kernel(rec, kind=REC_FUSED):
+    """Per kernel template (tile width, half-tile form or not), the
+    fused-pass launches of a record array: launches, summed ms, amplitudes
+    swept and terms (work items for batched launches)."""
+    rec = rec[rec["kind"] == kind]
+    half = (rec["direct"] & REC_HALF) != 0
     out = {}
     for t in np.unique(rec["tile_bits"]):
-        r = rec[rec["tile_bits"] == t]
-        out[int(t)] = {"launches": len(r), "ms": float(r["ms"].sum()),
-                       "amps": float(np.sum(2.0 ** r["nqb"].astype(float))),
-                       "terms": int(r["nterms"].sum())}
+        for h in (False, True):
+            r = rec[(rec["tile_bits"] == t) & (half == h)]
+            if len(r):
+                out[fused_name(int(t), kind == REC_BATCH, h)] = {
+                    "launches": len(r), "ms": float(r["ms"].sum()),
+                    "amps": float(np.sum(r["amps"].astype(float))),
+                    "terms": int(r["nterms"].sum())}
     return out
 
 
-def dominant_fused(rec, prof):
+def dominant_fused(rec, prof, kind=REC_FUSED):
     """(kernel name, stats) of the fused kernel template with the most
-    launch time, and the launch count per tile width.  Falls back to the
+    launch time, and the launch count per template.  Falls back to the
     aggregate when the records are empty."""
-    per = fused_by_tile(rec)
+    per = fused_by_kernel(rec, kind)
     if not per:
-        return None, prof["fused"], {}
-    tb = max(per, key=lambda t: per[t]["ms"])
-    return fused_name(tb), per[tb], {t: v["launches"] for t, v in per.items()}
+        return None, prof["fused" if kind == REC_FUSED else "batched"], {}
+    k = max(per, key=lambda n: per[n]["ms"])
+    return k, per[k], {n: v["launches"] for n, v in per.items()}
 
 
 def spread(n, k):
@@ -350,7 +360,7 @@ def bench_trott_n33(args, nat, costmodel, schedule):
                 "blocks_timed": blocks, "blocks_in_sweep": nblocks,
                 "delta": w["delta"]},
         extra={"terms_per_pass": fz["terms"] / max(1, fz["launches"]),
-               "launches_per_tile_width": tiles,
+               "launches_per_kernel": tiles,
                "first_visit_seconds": sched_s if schedule == "learned"
                else None},
         cpu=lambda ranks: cpu_n33_proxy(args, nat, ranks),
@@ -437,7 +447,7 @@ def bench_trott_n16(args, nat, costmodel, schedule):
         config={"workload": "trott1_n16_full", "nqb": 16, "terms": len(hm),
                 "steps": args.steps, "delta": w["delta"]},
         extra={"overlap_last": [float(out[-1].real), float(out[-1].imag)],
-               "launches_per_tile_width": tiles},
+               "launches_per_kernel": tiles},
         cpu=lambda ranks: cpu_circ(
             1, synth.trott_n16()["hm"], w["det_idx"], w["det_cf"], ranks,
             f"trott n=16, {args.steps} steps of {len(hm)} terms",
@@ -472,7 +482,7 @@ def bench_trott2_n26(args, nat, costmodel, schedule):
                 "determinants": 230230},
         extra={"terms_per_pass": prof["fused"]["terms"]
                / max(1, prof["fused"]["launches"]),
-               "launches_per_tile_width": tiles},
+               "launches_per_kernel": tiles},
         cpu=lambda ranks: cpu_rotations(
             26, xm[:256], zm[:256], 0.5 * w["delta"] * cf[:256], ranks,
             "first 256 rotations of a trott2 forward half-sweep"),
@@ -512,7 +522,7 @@ def bench_cmpsit_n30(args, nat, costmodel, schedule):
                 "angle_det": w["angle_det"], "angle_rand": w["angle_rand"]},
         extra={"terms_per_pass": prof["fused"]["terms"]
                / max(1, prof["fused"]["launches"]),
-               "launches_per_tile_width": tiles},
+               "launches_per_kernel": tiles},
         cpu=lambda ranks: cpu_rotations(
             30, dx[:16], dz[:16], phi[:16], ranks,
             "first 16 deterministic rotations of a cmpsit half-step"),
@@ -536,10 +546,13 @@ def bench_qdrift_batch(args, nat, costmodel, schedule):
     sub = [dict(r, samples=1) for r in regs[:64]]
     nat.prof_enable(True)
     nat.prof_read(reset=True)
+    nat.prof_records(reset=True)
     nat.batch_qdrift(sub, nat.BATCH_FUSED_BATCHED, st["tile_bits"],
                      st["low_bits"])
     prof = nat.prof_read(reset=True)
+    rec = nat.prof_records(reset=True)
     nat.prof_enable(False)
+    kern, kp, kernels = dominant_fused(rec, prof, REC_BATCH)
     _, sp = nat.batch_qdrift(sub, nat.BATCH_PLAIN_EACH, 12, args.low)
     _, sf = nat.batch_qdrift(sub, nat.BATCH_FUSED_EACH, st["tile_bits"],
                              st["low_bits"])
@@ -550,8 +563,7 @@ def bench_qdrift_batch(args, nat, costmodel, schedule):
         plain={"amp_updates_per_s": sp["amp_updates"] / sp["seconds"],
                "subset": "first 64 registers, 1 sample, one launch per "
                "rotation", "launches": sp["launches"]},
-        roofline=roofline(fused_name(st["tile_bits"], True),
-                          prof["batched"]),
+        roofline=roofline(kern or fused_name(st["tile_bits"], True), kp),
         config={"workload": "qdrift_batch_512", "registers": len(regs),
                 "widths": [int(min(widths)), int(max(widths))],
                 "samples": args.steps, "depth": 1000,
@@ -560,6 +572,7 @@ def bench_qdrift_batch(args, nat, costmodel, schedule):
                "launches": st["launches"], "passes": st["passes"],
                "tile_bits": st["tile_bits"], "low_bits": st["low_bits"],
                "tune_seconds": st["tune_seconds"],
+               "launches_per_kernel": kernels,
                "fused_each_amp_updates_per_s":
                    sf["amp_updates"] / sf["seconds"]},
         cpu=lambda ranks: cpu_qdrift_sample(regs, ranks),

@@ -96,6 +96,18 @@ int ph2hip_set_fused_form(uint32_t pipe_mask, uint64_t persistent_wgs);
  * Every setting gives the same state. */
 int ph2hip_set_fused_direct(int on);
 
+/* Bit tb of tile_mask: fused passes of tile width tb take the half-tile
+ * form where they have one (the exchanges between micro-groups run in two
+ * rounds through half a tile of LDS, so twice the workgroups fit a CU;
+ * first and last groups move their cosets between HBM and registers).
+ * Only tile 12 has that form (PH2HIP_HALF_TILES); other bits are an error.
+ * Default: the PHASE2_HIP_HALF environment variable, else
+ * PH2HIP_HALF_TILES.  Every
+ * setting gives the same state.  No counterpart in the reference (its
+ * CUDA backend has no fused pass). */
+#define PH2HIP_HALF_TILES (1u << 12)
+int ph2hip_set_fused_half(uint32_t tile_mask);
+
 
 /* Shape of the fused pass over rot[0..n) without launching it (shape has 4
  * entries): shape[0] = contiguous low bits L, shape[1] = gathered
@@ -207,16 +219,20 @@ int ph2hip_prof_read(struct ph2hip_prof *out, int reset);
 
 /* Per-launch records kept while profiling is enabled (the newest
  * PH2HIP_REC_MAX).  For fused passes: tile bits, contiguous low bits L,
- * gathered dimensions g, terms, micro-groups, which of the first (bit 0)
- * and last (bit 1) micro-groups move their cosets straight between HBM and
- * registers, the index bits the tile gathers (pivot mask), the union of the
- * pass's flip and phase masks, and the kernel time.  ph2hip_prof_records copies up to cap records (oldest
- * first, after draining pending events) and returns how many, or -1. */
+ * gathered dimensions g, terms, micro-groups, the form (direct: bit 0 and
+ * bit 1 when the first and last micro-groups move their cosets straight
+ * between HBM and registers, bit 2 for the half-tile form), the index bits
+ * the tile gathers (pivot mask), the union of the pass's flip and phase
+ * masks.  For every launch: the amplitudes it reads and writes once, and
+ * the kernel time.  For batched launches ngroups holds the number of work
+ * items and direct bit 2 the half-tile form.  ph2hip_prof_records copies up
+ * to cap records (oldest first, after draining pending events) and returns
+ * how many, or -1. */
 #define PH2HIP_REC_MAX 65536
 struct ph2hip_launch_rec {
 	uint32_t kind, nqb, tile_bits, low_bits;
 	uint32_t gdim, nterms, ngroups, direct;
-	uint64_t pivots, xunion, zunion;
+	uint64_t pivots, xunion, zunion, amps;
 	double ms;
 };
 long ph2hip_prof_records(struct ph2hip_launch_rec *out, size_t cap,

@@ -339,7 +339,8 @@ class Register:
 REC_DTYPE = np.dtype([
     ("kind", "<u4"), ("nqb", "<u4"), ("tile_bits", "<u4"), ("low_bits", "<u4"),
     ("gdim", "<u4"), ("nterms", "<u4"), ("ngroups", "<u4"), ("direct", "<u4"),
-    ("pivots", "<u8"), ("xunion", "<u8"), ("zunion", "<u8"), ("ms", "<f8")])
+    ("pivots", "<u8"), ("xunion", "<u8"), ("zunion", "<u8"), ("amps", "<u8"),
+    ("ms", "<f8")])
 
 
 def prof_records(reset=True, cap=65536) -> np.ndarray:
@@ -365,6 +366,15 @@ def set_fused_direct(on: bool):
     """First/last micro-groups straight between HBM and registers."""
     if lib().ph2hip_set_fused_direct(C.c_int(int(on))) != 0:
         _err("ph2hip_set_fused_direct")
+
+
+HALF_TILES = 1 << 12   # PH2HIP_HALF_TILES: the default half-tile mask
+
+
+def set_fused_half(tile_mask: int):
+    """Bit tb: fused passes of tile width tb in the half-tile LDS form."""
+    if lib().ph2hip_set_fused_half(C.c_uint32(tile_mask)) != 0:
+        _err("ph2hip_set_fused_half")
 
 
 def set_plain_form(form: int):

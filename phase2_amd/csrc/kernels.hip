@@ -264,7 +264,10 @@ struct alignas(16) FusedGroup {
 	};
 	union {
 		struct {
-			uint8_t first, count, pad2[2];
+			// terms first .. first + count; for a half-tile pass,
+			// the exchange to the next group splits the tile by tile
+			// bit xh, carried by coset-index bit xk
+			uint8_t first, count, xh, xk;
 		};
 		uint32_t fcw;
 	};
@@ -453,6 +456,80 @@ __device__ __forceinline__ uint64_t tile_base(const FusedPass &P, uint64_t t)
 	return base;
 }
 
+// The terms of one micro-group (P.t[first .. first + count), fcw) on one
+// coset of 2^D registers with base coordinate bu in the tile at base.
+template <int D>
+__device__ __forceinline__ void group_terms(double (&re)[1 << D],
+	double (&im)[1 << D], const FusedPass &P, uint32_t fcw, uint32_t bu,
+	uint64_t base)
+{
+	const uint32_t kfirst = fcw & 0xffu;
+	const uint32_t kend = kfirst + ((fcw >> 8) & 0xffu);
+	// term k + 1 is loaded (scalar loads) while term k runs
+	uint64_t tz = P.t[kfirst].z;
+	double c = P.t[kfirst].c, ts = P.t[kfirst].s;
+	uint32_t tzt = P.t[kfirst].zt, meta = P.t[kfirst].meta;
+	for (uint32_t k = kfirst; k < kend; k++) {
+		const uint32_t kn = k + 1 < kend ? k + 1 : k;
+		const uint64_t tz_n = P.t[kn].z;
+		const double c_n = P.t[kn].c, ts_n = P.t[kn].s;
+		const uint32_t tzt_n = P.t[kn].zt,
+			meta_n = P.t[kn].meta;
+		const uint32_t xtp = meta & 0xffu;
+		const uint32_t ztp = (meta >> 8) & 0xffu;
+		const uint32_t ty = (meta >> 16) & 0xffu;
+		const uint32_t bpt = (__popc(bu & tzt) ^
+			__popcll(base & tz)) & 1u;
+		if (xtp == 0) {
+			// diagonal: a <- c a + s_j (i a), s_j = +-s
+			const double sb = flip_sign(ts, bpt);
+#pragma unroll
+			for (int w = 0; w < (1 << D); w++)
+				diag_inplace(re[w], im[w], c, mul(sb,
+					unit_sign(__popc(w & ztp) & 1u)));
+		} else {
+			const double sb = flip_sign(ts,
+				bpt ^ ((ty >> 1) & 1u));
+			if (ty & 1)
+				coset_dispatch<D, true>(re, im, xtp,
+					ztp, sb, c);
+			else
+				coset_dispatch<D, false>(re, im, xtp,
+					ztp, sb, c);
+		}
+		tz = tz_n;
+		c = c_n;
+		ts = ts_n;
+		tzt = tzt_n;
+		meta = meta_n;
+	}
+}
+
+// Tile coordinate of coset j's base: j's bits spread over the free tile
+// bits jpos (three packed words).
+template <int TB, int D>
+__device__ __forceinline__ uint32_t coset_base(uint32_t j, uint32_t jw0,
+	uint32_t jw1, uint32_t jw2)
+{
+	const uint32_t jw[3] = { jw0, jw1, jw2 };
+	uint32_t bu = 0;
+#pragma unroll
+	for (int k = 0; k < TB - D; k++)
+		bu |= (j >> k & 1u) << ((jw[k >> 2] >> (8 * (k & 3))) & 0xffu);
+	return bu;
+}
+
+// HBM index of tile coordinate bu (row part ^ column part).
+__device__ __forceinline__ uint64_t hbm_index(const FusedPass &P,
+	uint64_t base, uint32_t bu, uint32_t L, uint32_t g)
+{
+	uint64_t abu = base ^ (bu & ((1u << L) - 1));
+	for (uint32_t b = 0; b < g; b++)
+		if ((bu >> (L + b)) & 1u)
+			abu ^= P.w[b];
+	return abu;
+}
+
 // All micro-groups of a pass on one tile.  Each group's cosets come from
 // and go back to the LDS tile, except that with dflags bit 0 the first
 // group reads its cosets straight from HBM (the tile was not staged) and
@@ -467,7 +544,7 @@ __device__ __forceinline__ void tile_groups(v2d *__restrict__ a, v2d *tile,
 	constexpr uint32_t NSUB = (1u << TB) >> D;    // cosets per group
 	static_assert(NSUB % kBlock == 0, "cosets must cover the block");
 	const uint32_t ngroups = pass_ngroups(P);
-	const uint32_t L = pass_L(P), g = pass_g(P), lmask = (1u << L) - 1;
+	const uint32_t L = pass_L(P), g = pass_g(P);
 	for (uint32_t gi = 0; gi < ngroups; gi++) {
 		const uint32_t uw0 = P.grp[gi].uw[0], uw1 = P.grp[gi].uw[1];
 		const uint32_t jw0 = P.grp[gi].jposw[0], jw1 = P.grp[gi].jposw[1],
@@ -481,32 +558,20 @@ __device__ __forceinline__ void tile_groups(v2d *__restrict__ a, v2d *tile,
 #pragma unroll
 		for (int i = 0; i < 4; i++)
 			gs[i] = lds_slot(gu[i]) << 4;
-		const uint32_t jw[3] = { jw0, jw1, jw2 };
 		const bool gload = gi == 0 && (dflags & 1u);
 		const bool gstore = gi + 1 == ngroups && (dflags & 2u);
 		if (!gload)
 			__syncthreads();
 		for (uint32_t j = tid; j < NSUB; j += kBlock) {
-			// coset base: j's bits spread over the free tile bits
-			uint32_t bu = 0;
-#pragma unroll
-			for (int k = 0; k < TB - D; k++)
-				bu |= (j >> k & 1u) <<
-					((jw[k >> 2] >> (8 * (k & 3))) & 0xffu);
+			const uint32_t bu = coset_base<TB, D>(j, jw0, jw1, jw2);
 			// byte offset of the coset base's slot: the per-element
 			// offsets (uniform) are XORed in pre-shifted, one VALU
 			// operation per LDS access
 			uint32_t sbu16 = lds_slot(bu) << 4;
 			__asm__("" : "+v"(sbu16));   // keeps the shift out of
 			                             // the per-access XOR
-			// HBM index of coordinate bu (row part ^ column part)
-			uint64_t abu = 0;
-			if (gload || gstore) {
-				abu = base ^ (bu & lmask);
-				for (uint32_t b = 0; b < g; b++)
-					if ((bu >> (L + b)) & 1u)
-						abu ^= P.w[b];
-			}
+			const uint64_t abu = (gload || gstore) ?
+				hbm_index(P, base, bu, L, g) : 0;
 			double re[1 << D], im[1 << D];
 			if (gload) {
 #pragma unroll
@@ -530,46 +595,7 @@ __device__ __forceinline__ void tile_groups(v2d *__restrict__ a, v2d *tile,
 					im[w] = v.y;
 				}
 			}
-			const uint32_t kfirst = fcw & 0xffu;
-			const uint32_t kend = kfirst + ((fcw >> 8) & 0xffu);
-			// term k + 1 is loaded (scalar loads) while term k runs
-			uint64_t tz = P.t[kfirst].z;
-			double c = P.t[kfirst].c, ts = P.t[kfirst].s;
-			uint32_t tzt = P.t[kfirst].zt, meta = P.t[kfirst].meta;
-			for (uint32_t k = kfirst; k < kend; k++) {
-				const uint32_t kn = k + 1 < kend ? k + 1 : k;
-				const uint64_t tz_n = P.t[kn].z;
-				const double c_n = P.t[kn].c, ts_n = P.t[kn].s;
-				const uint32_t tzt_n = P.t[kn].zt,
-					meta_n = P.t[kn].meta;
-				const uint32_t xtp = meta & 0xffu;
-				const uint32_t ztp = (meta >> 8) & 0xffu;
-				const uint32_t ty = (meta >> 16) & 0xffu;
-				const uint32_t bpt = (__popc(bu & tzt) ^
-					__popcll(base & tz)) & 1u;
-				if (xtp == 0) {
-					// diagonal: a <- c a + s_j (i a), s_j = +-s
-					const double sb = flip_sign(ts, bpt);
-#pragma unroll
-					for (int w = 0; w < (1 << D); w++)
-						diag_inplace(re[w], im[w], c, mul(sb,
-							unit_sign(__popc(w & ztp) & 1u)));
-				} else {
-					const double sb = flip_sign(ts,
-						bpt ^ ((ty >> 1) & 1u));
-					if (ty & 1)
-						coset_dispatch<D, true>(re, im, xtp,
-							ztp, sb, c);
-					else
-						coset_dispatch<D, false>(re, im, xtp,
-							ztp, sb, c);
-				}
-				tz = tz_n;
-				c = c_n;
-				ts = ts_n;
-				tzt = tzt_n;
-				meta = meta_n;
-			}
+			group_terms<D>(re, im, P, fcw, bu, base);
 			if (gstore) {
 #pragma unroll
 				for (int w = 0; w < (1 << D); w++) {
@@ -598,6 +624,118 @@ __device__ __forceinline__ void tile_groups(v2d *__restrict__ a, v2d *tile,
 	}
 	if (!(dflags & 2u))
 		__syncthreads();
+}
+
+// Slot of tile coordinate v in a buffer of half a tile: bits h and TB-1
+// exchanged, the whole-tile slot map, bit TB-1 dropped.  Linear, and one-
+// to-one on each half {v : bit h of v = const}.
+template <int TB>
+__device__ __forceinline__ uint32_t half_slot(uint32_t v, uint32_t h)
+{
+	const uint32_t t = ((v >> h) ^ (v >> (TB - 1))) & 1u;
+	v ^= t * ((1u << h) | (1u << (TB - 1)));
+	return lds_slot(v) & ((1u << (TB - 1)) - 1);
+}
+
+// Byte offsets in the half buffer of a group's basis vectors (uniform).
+template <int TB>
+__device__ __forceinline__ void half_offsets(const FusedGroup &G, uint32_t h,
+	uint32_t (&gs)[4])
+{
+	const uint32_t uw0 = G.uw[0], uw1 = G.uw[1];
+	gs[0] = half_slot<TB>(uw0 & 0xffffu, h) << 4;
+	gs[1] = half_slot<TB>(uw0 >> 16, h) << 4;
+	gs[2] = half_slot<TB>(uw1 & 0xffffu, h) << 4;
+	gs[3] = half_slot<TB>(uw1 >> 16, h) << 4;
+}
+
+// All micro-groups of a pass on one tile with half a tile of LDS: the
+// first group reads its cosets from HBM, the last writes them to HBM, and
+// between groups q and q+1 the cosets move through the buffer in two
+// rounds, one per half of the tile (split bit xh of group q; the threads
+// of one half are the waves whose coset-index bit xk is that half).  One
+// coset per thread.
+template <int TB, int D>
+__device__ __forceinline__ void tile_groups_half(v2d *__restrict__ a,
+	v2d *buf, const FusedPass &P, uint64_t base, uint32_t tid)
+{
+	static_assert(((1u << TB) >> D) == kBlock, "one coset per thread");
+	const uint32_t ngroups = pass_ngroups(P);
+	const uint32_t L = pass_L(P), g = pass_g(P);
+	uint32_t bu = coset_base<TB, D>(tid, P.grp[0].jposw[0],
+		P.grp[0].jposw[1], P.grp[0].jposw[2]);
+	double re[1 << D], im[1 << D];
+	{
+		const uint64_t abu = hbm_index(P, base, bu, L, g);
+#pragma unroll
+		for (int w = 0; w < (1 << D); w++) {
+			const v2d v = a[abu ^ P.dcw[0][w]];
+			re[w] = v.x;
+			im[w] = v.y;
+		}
+	}
+	for (uint32_t gi = 0;; gi++) {
+		const uint32_t fcw = P.grp[gi].fcw;
+		group_terms<D>(re, im, P, fcw, bu, base);
+		if (gi + 1 == ngroups)
+			break;
+		const FusedGroup &N = P.grp[gi + 1];
+		const uint32_t h = (fcw >> 16) & 0xffu, xk = fcw >> 24;
+		const uint32_t half = __builtin_amdgcn_readfirstlane(
+			(tid >> xk) & 1u);
+		const uint32_t bn = coset_base<TB, D>(tid, N.jposw[0],
+			N.jposw[1], N.jposw[2]);
+		uint32_t gw[4], gr[4];
+		half_offsets<TB>(P.grp[gi], h, gw);
+		half_offsets<TB>(N, h, gr);
+		uint32_t sw = half_slot<TB>(bu, h) << 4;
+		uint32_t sr = half_slot<TB>(bn, h) << 4;
+		__asm__("" : "+v"(sw), "+v"(sr));
+		for (uint32_t q = 0; q < 2; q++) {
+			__syncthreads();   // the buffer's previous readers are done
+			if (half == q) {
+#pragma unroll
+				for (int w = 0; w < (1 << D); w++) {
+					uint32_t c16 = 0;
+#pragma unroll
+					for (int i = 0; i < D; i++)
+						if (w >> i & 1)
+							c16 ^= gw[i];
+					v2d v;
+					v.x = re[w];
+					v.y = im[w];
+					*reinterpret_cast<v2d *>(
+						reinterpret_cast<char *>(buf) +
+						(sw ^ c16)) = v;
+				}
+			}
+			__syncthreads();
+			if (half == q) {
+#pragma unroll
+				for (int w = 0; w < (1 << D); w++) {
+					uint32_t c16 = 0;
+#pragma unroll
+					for (int i = 0; i < D; i++)
+						if (w >> i & 1)
+							c16 ^= gr[i];
+					const v2d v = *reinterpret_cast<const v2d *>(
+						reinterpret_cast<const char *>(buf) +
+						(sr ^ c16));
+					re[w] = v.x;
+					im[w] = v.y;
+				}
+			}
+		}
+		bu = bn;
+	}
+	const uint64_t abu = hbm_index(P, base, bu, L, g);
+#pragma unroll
+	for (int w = 0; w < (1 << D); w++) {
+		v2d v;
+		v.x = re[w];
+		v.y = im[w];
+		a[abu ^ P.dcw[1][w]] = v;
+	}
 }
 
 // Direct HBM->LDS tile loads (gfx950 global_load_lds_dwordx4): the data
@@ -712,6 +850,21 @@ __global__ __launch_bounds__(kBlock) void k_rot_fused(
 	fused_tile<TB, D>(reinterpret_cast<v2d *>(amp), tile, P, t);
 }
 
+// Half-tile form of k_rot_fused (descriptor flag 4): half the LDS, so
+// twice the workgroups per CU.
+template <int TB, int D>
+__global__ __launch_bounds__(kBlock) void k_rot_fused_half(
+	double2 *__restrict__ amp, const FusedPass P, uint64_t ntiles)
+{
+	__shared__ v2d buf[1u << (TB - 1)];
+	const uint64_t w = (uint64_t)blockIdx.y * gridDim.x + blockIdx.x;
+	if (w >= ntiles)
+		return;           // whole workgroup: no barrier is left waiting
+	const uint64_t t = xcd_tile(w, ntiles);
+	tile_groups_half<TB, D>(reinterpret_cast<v2d *>(amp), buf, P,
+		tile_base(P, t), threadIdx.x);
+}
+
 // Batched pass: pass p of many registers in one launch.  Work item i covers
 // tiles [tile0_i, tile0_{i+1}) of register i (amplitudes at amp_off_i) with
 // descriptor desc_i; a workgroup finds its item by a uniform binary search.
@@ -721,12 +874,14 @@ struct BatchItem {
 	uint64_t desc;
 };
 
-template <int TB, int D>
+// HALF: every descriptor of the launch has the half-tile form (half the
+// LDS, as k_rot_fused_half).
+template <int TB, int D, bool HALF = false>
 __global__ __launch_bounds__(kBlock) void k_rot_fused_batch(
 	double2 *__restrict__ amp, const FusedPass *__restrict__ descs,
 	const BatchItem *__restrict__ items, uint32_t nitems, uint64_t ntiles)
 {
-	__shared__ v2d tile[1u << TB];
+	__shared__ v2d tile[1u << (HALF ? TB - 1 : TB)];
 	// tile order kept round-robin over the XCDs here: the registers of a
 	// batch differ in work per tile, and interleaving balances the XCDs
 	// (contiguous ranges measured 8% slower on the 512-register batch)
@@ -742,13 +897,16 @@ __global__ __launch_bounds__(kBlock) void k_rot_fused_batch(
 			hi = mid;
 	}
 	const BatchItem it = items[lo];
-	fused_tile<TB, D>(reinterpret_cast<v2d *>(amp) + it.amp_off, tile,
-		descs[it.desc], t - it.tile0);
+	if constexpr (HALF) {
+		const FusedPass &P = descs[it.desc];
+		tile_groups_half<TB, D>(reinterpret_cast<v2d *>(amp) + it.amp_off,
+			tile, P, tile_base(P, t - it.tile0), threadIdx.x);
+	} else {
+		fused_tile<TB, D>(reinterpret_cast<v2d *>(amp) + it.amp_off, tile,
+			descs[it.desc], t - it.tile0);
+	}
 }
 
-// Persistent, software-pipelined form: each workgroup walks tiles
-// blockIdx.x, +gridDim.x, ...; the next tile's HBM reads are in flight in
-// registers while the current tile is rotated in LDS.
 // Persistent, software-pipelined pass: each workgroup walks tiles
 // blockIdx.x, +gridDim.x, ...; the next tile's HBM reads are in flight in
 // registers while the current tile is rotated in LDS, and its writes are
@@ -1008,6 +1166,7 @@ int prof_begin(int kind, uint32_t nterms, uint64_t amps, hipEvent_t *stop,
 		g_pmeta[g_pn] = ph2hip_launch_rec{};
 	g_pmeta[g_pn].kind = (uint32_t)kind;
 	g_pmeta[g_pn].nterms = nterms;
+	g_pmeta[g_pn].amps = amps;
 	HIPCHK(hipEventRecord(g_pev[g_pn][0], g_stream));
 	*stop = g_pev[g_pn][1];
 	g_pn++;
@@ -1082,6 +1241,12 @@ constexpr int group_depth(int tb)
 	return tb >= 12 ? 4 : tb - 8;
 }
 
+// Tile widths with a half-tile form (it needs one coset per thread,
+// 2^(tb - D) = 256).  Only tile 12 gains from it: there the whole-tile LDS
+// (64 KiB) allows two workgroups per CU and the half form four; narrower
+// tiles already reach the register-limited count.
+constexpr uint32_t kHalfTiles = PH2HIP_HALF_TILES;
+
 bool g_pipe_init = false;
 uint32_t g_pipe_mask = 0;   // bit TB set: persistent pipelined form
 bool g_direct = true;       // first/last groups straight to and from HBM
@@ -1119,6 +1284,16 @@ int launch_fused(double2 *a, uint32_t nqb, const FusedPass &P,
 	} else {
 		const uint64_t gx = ntiles < kMaxGrid ? ntiles : kMaxGrid;
 		const uint64_t gy = (ntiles + gx - 1) / gx;
+		if constexpr ((kHalfTiles >> TB & 1) != 0) {
+			if (P.dflags & 4u) {
+				hipLaunchKernelGGL(
+					(k_rot_fused_half<TB, group_depth(TB)>),
+					dim3((uint32_t)gx, (uint32_t)gy), dim3(kBlock),
+					0, g_stream, a, P, ntiles);
+				HIPCHK(hipGetLastError());
+				return prof_end(stop);
+			}
+		}
 		hipLaunchKernelGGL((k_rot_fused<TB, group_depth(TB)>),
 			dim3((uint32_t)gx, (uint32_t)gy), dim3(kBlock), 0, g_stream,
 			a, P, ntiles);
@@ -1175,9 +1350,10 @@ int plain_one(double2 *a, uint32_t nqb, const struct ph2hip_rot &r)
 
 // Descriptor of one fused pass over terms r[0..len) (global term index k0
 // for messages) in tiles of 2^tb amplitudes.  0, or -1 if the pass does not
-// fit its tile.
-int build_pass(FusedPass &P, const struct ph2hip_rot *r, size_t len,
-	uint32_t tb, uint32_t low_bits, size_t k0)
+// fit its tile; with half set, 1 if the pass has no half-tile form (some
+// exchange between consecutive groups has no split bit).
+int build_pass_mode(FusedPass &P, const struct ph2hip_rot *r, size_t len,
+	uint32_t tb, uint32_t low_bits, size_t k0, bool half)
 {
 	memset(&P, 0, sizeof P);
 	// Grow the contiguous low part L while the gathered rank g fits:
@@ -1241,17 +1417,75 @@ int build_pass(FusedPass &P, const struct ph2hip_rot *r, size_t len,
 	// micro-groups: consecutive terms whose xt span <= D dims
 	const uint32_t D = (uint32_t)group_depth((int)tb);
 	uint32_t ng = 0;
-	size_t k = 0;
-	while (k < len) {
+	uint32_t gk[PH2HIP_PASS_MAX + 1];   // group boundaries
+	uint32_t sup[PH2HIP_PASS_MAX];      // tile bits touched by each span
+	for (size_t k = 0; k < len;) {
 		size_t e = k + 1;
 		uint32_t u[8];
 		while (e < len && reduce32(xt + k, e + 1 - k, u, D) <= D)
 			e++;
+		const uint32_t d = reduce32(xt + k, e - k, u, D);
+		uint32_t sp = 0;
+		for (uint32_t m = 0; m < (1u << d); m++) {
+			uint32_t v = 0;
+			for (uint32_t i = 0; i < d; i++)
+				if (m >> i & 1)
+					v ^= u[i];
+			sp |= v;
+		}
+		gk[ng] = (uint32_t)k;
+		sup[ng++] = sp;
+		k = e;
+	}
+	gk[ng] = (uint32_t)len;
+	// Half-tile exchanges: between groups q and q+1 the tile moves through
+	// LDS in two halves split by tile bit xh[q], which no vector of either
+	// span touches, so that every coset lies in one half; coset-index bit
+	// xk[q] (a wave-select bit, 6 or 7) maps to xh[q] in both groups, so
+	// that a thread writes and reads in the same half.
+	uint8_t xh[PH2HIP_PASS_MAX] = {}, xk[PH2HIP_PASS_MAX] = {};
+	if (half) {
+		for (uint32_t q = 0; q + 1 < ng; q++) {
+			const uint32_t fr = ((1u << tb) - 1) & ~(sup[q] | sup[q + 1]);
+			if (!fr)
+				return 1;   // no split bit: not a half-tile pass
+			uint32_t h;
+			if (q > 0 && (fr >> xh[q - 1] & 1)) {
+				h = xh[q - 1];
+			} else {
+				// highest free bit above the lane bits 0..3
+				const uint32_t hi = fr & ~15u;
+				h = 31 - __builtin_clz(hi ? hi : fr);
+			}
+			xh[q] = (uint8_t)h;
+			xk[q] = (uint8_t)(q == 0 || h == xh[q - 1] ?
+				(q ? xk[q - 1] : 7) : 13 - xk[q - 1]);
+		}
+	}
+	for (uint32_t gi = 0; gi < ng; gi++) {
+		const size_t k = gk[gi], e = gk[gi + 1];
+		// tile bits and coset-index bits reserved for the exchanges in
+		// and out of this group
+		uint32_t rbits = 0;
+		int rpos[2] = { -1, -1 }, rbit[2] = { -1, -1 };
+		if (half && gi > 0) {
+			rbits |= 1u << xh[gi - 1];
+			rpos[0] = xk[gi - 1];
+			rbit[0] = xh[gi - 1];
+		}
+		if (half && gi + 1 < ng) {
+			rbits |= 1u << xh[gi];
+			rpos[1] = xk[gi];
+			rbit[1] = xh[gi];
+		}
+		uint32_t u[8];
 		uint32_t d = reduce32(xt + k, e - k, u, D);
 		// pad with unit vectors of tile bits outside the span, highest
 		// first, so that tile bits 0..3 stay free for the lanes where
 		// possible (whole 256-byte runs per quarter-wave)
 		for (uint32_t bit = tb; d < D && bit-- > 0;) {
+			if (rbits >> bit & 1)
+				continue;
 			uint32_t x = 1u << bit;
 			for (uint32_t i = 0; i < d && x; i++)
 				if (x >> (31 - __builtin_clz(u[i])) & 1)
@@ -1264,6 +1498,8 @@ int build_pass(FusedPass &P, const struct ph2hip_rot *r, size_t len,
 					u[i] ^= x;
 			u[d++] = x;
 		}
+		if (d < D)
+			return half ? 1 : fail("rotate_fused: group padding");
 		for (uint32_t i = 1; i < d; i++)   // ascending pivots
 			for (uint32_t j = i; j > 0 && __builtin_clz(u[j - 1]) <
 				__builtin_clz(u[j]); j--) {
@@ -1271,7 +1507,7 @@ int build_pass(FusedPass &P, const struct ph2hip_rot *r, size_t len,
 				u[j] = u[j - 1];
 				u[j - 1] = t;
 			}
-		FusedGroup &G = P.grp[ng++];
+		FusedGroup &G = P.grp[gi];
 		memset(&G, 0, sizeof G);
 		for (uint32_t i = 0; i < D; i++) {
 			G.u[i] = (uint16_t)u[i];
@@ -1280,28 +1516,43 @@ int build_pass(FusedPass &P, const struct ph2hip_rot *r, size_t len,
 		// Coset index bits -> free tile bits.  The first four (lanes
 		// 0..15 of a wave) take free bits whose LDS-slot images
 		// (tile bit t < 12 lands on slot bit t mod 4) are distinct,
-		// so a quarter-wave reads 16 different bank groups.
+		// so a quarter-wave reads 16 different bank groups.  Reserved
+		// index bits take their exchange split bits.
 		{
-			uint32_t used = 0, npos = 0;
+			uint32_t used = rbits, npos = 0;
 			for (uint32_t i = 0; i < D; i++)
 				used |= 1u << G.piv[i];
+			bool taken[16] = {};
+			for (int r = 0; r < 2; r++)
+				if (rpos[r] >= 0) {
+					G.jpos[rpos[r]] = (uint8_t)rbit[r];
+					taken[rpos[r]] = true;
+				}
 			for (uint32_t res = 0; res < 4; res++)
 				for (uint32_t t = res; t < tb && t < 12; t += 4)
 					if (!(used >> t & 1)) {
 						used |= 1u << t;
+						while (taken[npos])
+							npos++;
 						G.jpos[npos++] = (uint8_t)t;
 						break;
 					}
 			for (uint32_t t = 0; t < tb; t++)
 				if (!(used >> t & 1)) {
 					used |= 1u << t;
+					while (taken[npos])
+						npos++;
 					G.jpos[npos++] = (uint8_t)t;
 				}
-			if (npos != tb - D)
+			while (npos < 16 && taken[npos])
+				npos++;
+			if (npos != tb - D || used != (1u << tb) - 1)
 				return fail("rotate_fused: coset map");
 		}
 		G.first = (uint8_t)k;
 		G.count = (uint8_t)(e - k);
+		G.xh = xh[gi];
+		G.xk = xk[gi];
 		for (size_t q = k; q < e; q++) {
 			uint32_t xp = 0, zp = 0, rest = xt[q];
 			for (uint32_t i = 0; i < D; i++) {
@@ -1317,16 +1568,16 @@ int build_pass(FusedPass &P, const struct ph2hip_rot *r, size_t len,
 			P.t[q].xtp = (uint8_t)xp;
 			P.t[q].ztp = (uint8_t)zp;
 		}
-		k = e;
 	}
 	P.ngroups = (uint8_t)ng;
 	// First and last group straight between HBM and registers when their
-	// coset-index bits 0..3 are tile bits 0..3 inside rows of >= 16.
-	if (g_direct && L >= 4) {
+	// coset-index bits 0..3 are tile bits 0..3 inside rows of >= 16; for a
+	// half-tile pass always (the tile is never staged whole in LDS).
+	if ((g_direct && L >= 4) || half) {
 		for (int side = 0; side < 2; side++) {
 			const FusedGroup &G = P.grp[side ? ng - 1 : 0];
-			if (G.jpos[0] != 0 || G.jpos[1] != 1 || G.jpos[2] != 2 ||
-			    G.jpos[3] != 3)
+			if (!half && (G.jpos[0] != 0 || G.jpos[1] != 1 ||
+			    G.jpos[2] != 2 || G.jpos[3] != 3))
 				continue;
 			for (uint32_t e = 0; e < (1u << D); e++) {
 				uint32_t cw = 0;
@@ -1342,7 +1593,41 @@ int build_pass(FusedPass &P, const struct ph2hip_rot *r, size_t len,
 			P.dflags |= 1u << side;
 		}
 	}
+	if (half)
+		P.dflags |= 4u;
 	return 0;
+}
+
+// Bit tb set: passes of tile width tb take the half-tile form where they
+// have one (k_rot_fused_half), else the whole-tile form.  Default: every
+// width that has the form (measured at 33 qubits, tile 12: 53.9 ms per
+// pass against 61.6 ms for the same greedy passes in the whole-tile form).
+uint32_t g_half_mask = kHalfTiles;
+bool g_half_init = false;
+
+uint32_t half_mask()
+{
+	if (!g_half_init) {
+		const char *e = getenv("PHASE2_HIP_HALF");
+		if (e)
+			g_half_mask = (uint32_t)strtoul(e, nullptr, 0) &
+				kHalfTiles;
+		g_half_init = true;
+	}
+	return g_half_mask;
+}
+
+// A single-register pass in the form its launch will take.
+int build_pass(FusedPass &P, const struct ph2hip_rot *r, size_t len,
+	uint32_t tb, uint32_t low_bits, size_t k0)
+{
+	if (half_mask() >> tb & 1) {
+		const int rc = build_pass_mode(P, r, len, tb, low_bits, k0,
+			true);
+		if (rc <= 0)
+			return rc;
+	}
+	return build_pass_mode(P, r, len, tb, low_bits, k0, false);
 }
 
 // Grow-only device buffer.
@@ -1366,10 +1651,20 @@ size_t g_bdesc_cap = 0, g_bitem_cap = 0;
 
 template <int TB>
 int launch_batch(double2 *amp, const FusedPass *descs, const BatchItem *items,
-	uint32_t nitems, uint64_t ntiles)
+	uint32_t nitems, uint64_t ntiles, bool half)
 {
 	const uint64_t gx = ntiles < kMaxGrid ? ntiles : kMaxGrid;
 	const uint64_t gy = (ntiles + gx - 1) / gx;
+	if constexpr ((kHalfTiles >> TB & 1) != 0) {
+		if (half) {
+			hipLaunchKernelGGL(
+				(k_rot_fused_batch<TB, group_depth(TB), true>),
+				dim3((uint32_t)gx, (uint32_t)gy), dim3(kBlock), 0,
+				g_stream, amp, descs, items, nitems, ntiles);
+			HIPCHK(hipGetLastError());
+			return 0;
+		}
+	}
 	hipLaunchKernelGGL((k_rot_fused_batch<TB, group_depth(TB)>),
 		dim3((uint32_t)gx, (uint32_t)gy), dim3(kBlock), 0, g_stream, amp,
 		descs, items, nitems, ntiles);
@@ -1829,6 +2124,16 @@ int ph2hip_set_fused_direct(int on)
 	return 0;
 }
 
+int ph2hip_set_fused_half(uint32_t tile_mask)
+{
+	half_mask();   // the environment default is read once, then replaced
+	if (tile_mask & ~kHalfTiles)
+		return fail("set_fused_half: mask 0x%x outside 0x%x", tile_mask,
+			kHalfTiles);
+	g_half_mask = tile_mask;
+	return 0;
+}
+
 int ph2hip_mark(int slot)
 {
 	if (ensure_init())
@@ -1879,8 +2184,10 @@ struct ph2hip_batch_plan {
 	FusedPass *desc;      // descriptors, job-major
 	BatchItem *items;     // work items, launch-major
 	size_t ndesc, maxp;
-	size_t *lstart;       // launch p: items [lstart[p], lstart[p+1])
-	uint64_t *ltiles;     // tiles of launch p
+	size_t nlaunch;       // one or two per pass index (half-tile, whole)
+	size_t *lstart;       // launch l: items [lstart[l], lstart[l+1])
+	uint64_t *ltiles;     // tiles of launch l
+	uint8_t *lhalf;       // launch l in the half-tile form
 	uint32_t tile_bits;
 	uint64_t namp_total;
 	hipGraph_t graph;
@@ -1899,6 +2206,7 @@ void ph2hip_batch_plan_free(struct ph2hip_batch_plan *pl)
 	free(pl->items);
 	free(pl->lstart);
 	free(pl->ltiles);
+	free(pl->lhalf);
 	free(pl);
 }
 
@@ -1945,11 +2253,13 @@ int ph2hip_batch_plan_build(const struct ph2hip_batch_job *jobs, size_t njobs,
 	}
 	pl->desc = (FusedPass *)malloc(sizeof(FusedPass) * ndesc);
 	pl->items = (BatchItem *)malloc(sizeof(BatchItem) * ndesc);
-	pl->lstart = (size_t *)calloc(maxp + 1, sizeof(size_t));
-	pl->ltiles = (uint64_t *)calloc(maxp, sizeof(uint64_t));
+	pl->lstart = (size_t *)calloc(2 * maxp + 1, sizeof(size_t));
+	pl->ltiles = (uint64_t *)calloc(2 * maxp, sizeof(uint64_t));
+	pl->lhalf = (uint8_t *)calloc(2 * maxp, 1);
 	size_t *first = (size_t *)malloc(sizeof(size_t) * (njobs ? njobs : 1));
 	int rc = 0;
-	if (!pl->desc || !pl->items || !pl->lstart || !pl->ltiles || !first) {
+	if (!pl->desc || !pl->items || !pl->lstart || !pl->ltiles ||
+	    !pl->lhalf || !first) {
 		rc = fail("rotate_batched: out of host memory");
 	} else {
 		// descriptors, job-major
@@ -1958,31 +2268,51 @@ int ph2hip_batch_plan_build(const struct ph2hip_batch_job *jobs, size_t njobs,
 			first[j] = d;
 			size_t k0 = 0;
 			for (size_t p = 0; p < jobs[j].npass && !rc; p++) {
-				rc = build_pass(pl->desc[d++], jobs[j].rot + k0,
-					jobs[j].pass_len[p], tile_bits, low_bits, k0);
+				rc = 1;
+				if (half_mask() >> tile_bits & 1)
+					rc = build_pass_mode(pl->desc[d],
+						jobs[j].rot + k0, jobs[j].pass_len[p],
+						tile_bits, low_bits, k0, true);
+				if (rc > 0)
+					rc = build_pass_mode(pl->desc[d],
+						jobs[j].rot + k0, jobs[j].pass_len[p],
+						tile_bits, low_bits, k0, false);
+				d++;
 				k0 += jobs[j].pass_len[p];
 			}
 		}
 	}
 	if (!rc) {
-		// work items, launch-major: launch p holds pass p of every job
-		// that has one
-		size_t it = 0;
+		// work items, launch-major: pass p of every job that has one,
+		// those in the half-tile form in one launch and the others in a
+		// second (the two kernels differ in LDS size)
+		size_t it = 0, l = 0;
 		for (size_t p = 0; p < maxp; p++) {
-			pl->lstart[p] = it;
-			uint64_t t0 = 0;
-			for (size_t j = 0; j < njobs; j++) {
-				if (jobs[j].npass <= p)
+			for (int h = 1; h >= 0; h--) {
+				const size_t it0 = it;
+				uint64_t t0 = 0;
+				for (size_t j = 0; j < njobs; j++) {
+					if (jobs[j].npass <= p)
+						continue;
+					const size_t dj = first[j] + p;
+					if (((pl->desc[dj].dflags & 4u) != 0) != (h != 0))
+						continue;
+					pl->items[it].amp_off = jobs[j].amp_off;
+					pl->items[it].tile0 = t0;
+					pl->items[it].desc = dj;
+					t0 += UINT64_C(1) << (jobs[j].nqb - tile_bits);
+					it++;
+				}
+				if (it == it0)
 					continue;
-				pl->items[it].amp_off = jobs[j].amp_off;
-				pl->items[it].tile0 = t0;
-				pl->items[it].desc = first[j] + p;
-				t0 += UINT64_C(1) << (jobs[j].nqb - tile_bits);
-				it++;
+				pl->lstart[l] = it0;
+				pl->ltiles[l] = t0;
+				pl->lhalf[l] = (uint8_t)h;
+				l++;
 			}
-			pl->ltiles[p] = t0;
 		}
-		pl->lstart[maxp] = it;
+		pl->lstart[l] = it;
+		pl->nlaunch = l;
 	}
 	free(first);
 	if (rc) {
@@ -2002,7 +2332,7 @@ int ph2hip_batch_plan_launch(void *amp, struct ph2hip_batch_plan *pl,
 		*launches = 0;
 	if (pl->ndesc == 0)
 		return 0;
-	const size_t ndesc = pl->ndesc, maxp = pl->maxp;
+	const size_t ndesc = pl->ndesc, nlaunch = pl->nlaunch;
 	const uint32_t tile_bits = pl->tile_bits;
 	// The upload follows the work already queued on the stream, so the
 	// launches of an earlier plan have read the buffers before they change.
@@ -2021,27 +2351,29 @@ int ph2hip_batch_plan_launch(void *amp, struct ph2hip_batch_plan *pl,
 	if (graph && hipStreamBeginCapture(g_stream,
 		    hipStreamCaptureModeThreadLocal) != hipSuccess)
 		return fail("rotate_batched: begin capture");
-	for (size_t p = 0; p < maxp && !rc; p++) {
-		const uint32_t ni = (uint32_t)(pl->lstart[p + 1] - pl->lstart[p]);
-		const BatchItem *it = (const BatchItem *)g_bitem + pl->lstart[p];
+	for (size_t l = 0; l < nlaunch && !rc; l++) {
+		const uint32_t ni = (uint32_t)(pl->lstart[l + 1] - pl->lstart[l]);
+		const BatchItem *it = (const BatchItem *)g_bitem + pl->lstart[l];
 		const FusedPass *dd = (const FusedPass *)g_bdesc;
+		const bool hf = pl->lhalf[l] != 0;
 		hipEvent_t stop = nullptr;
 		ph2hip_launch_rec meta{};
 		meta.tile_bits = tile_bits;
 		meta.ngroups = ni;
-		if (!graph && prof_begin(3, ni, pl->ltiles[p] << tile_bits, &stop,
+		meta.direct = hf ? 4u : 0u;
+		if (!graph && prof_begin(3, ni, pl->ltiles[l] << tile_bits, &stop,
 			    &meta)) {
 			rc = -1;
 			break;
 		}
 		double2 *a = (double2 *)amp;
-		const uint64_t nt = pl->ltiles[p];
+		const uint64_t nt = pl->ltiles[l];
 		switch (tile_bits) {
-		case 9: rc = launch_batch<9>(a, dd, it, ni, nt); break;
-		case 10: rc = launch_batch<10>(a, dd, it, ni, nt); break;
-		case 11: rc = launch_batch<11>(a, dd, it, ni, nt); break;
-		case 12: rc = launch_batch<12>(a, dd, it, ni, nt); break;
-		default: rc = launch_batch<13>(a, dd, it, ni, nt); break;
+		case 9: rc = launch_batch<9>(a, dd, it, ni, nt, hf); break;
+		case 10: rc = launch_batch<10>(a, dd, it, ni, nt, hf); break;
+		case 11: rc = launch_batch<11>(a, dd, it, ni, nt, hf); break;
+		case 12: rc = launch_batch<12>(a, dd, it, ni, nt, hf); break;
+		default: rc = launch_batch<13>(a, dd, it, ni, nt, hf); break;
 		}
 		if (!rc && !graph && prof_end(stop))
 			rc = -1;
@@ -2058,7 +2390,7 @@ int ph2hip_batch_plan_launch(void *amp, struct ph2hip_batch_plan *pl,
 			rc = fail("rotate_batched: graph launch");
 	}
 	if (!rc && launches)
-		*launches = maxp;
+		*launches = nlaunch;
 	return rc;
 }
 

@@ -18,14 +18,17 @@ from phase2_amd import _native  # noqa: E402
 def test_fused_name():
     assert bench.fused_name(12) == "k_rot_fused<12, 4>"
     assert bench.fused_name(11) == "k_rot_fused<11, 3>"
-    assert bench.fused_name(10, True) == "k_rot_fused_batch<10, 2>"
+    assert bench.fused_name(10, True) == "k_rot_fused_batch<10, 2, false>"
+    assert bench.fused_name(12, True, True) == \
+        "k_rot_fused_batch<12, 4, true>"
+    assert bench.fused_name(12, half=True) == "k_rot_fused_half<12, 4>"
 
 
 def _recs(rows):
     r = np.zeros(len(rows), dtype=_native.REC_DTYPE)
     for i, (kind, nqb, tb, nt, ms) in enumerate(rows):
         r[i]["kind"], r[i]["nqb"], r[i]["tile_bits"] = kind, nqb, tb
-        r[i]["nterms"], r[i]["ms"] = nt, ms
+        r[i]["nterms"], r[i]["ms"], r[i]["amps"] = nt, ms, 1 << nqb
     return r
 
 
@@ -36,7 +39,7 @@ def test_dominant_fused_by_time_not_count():
                 + [(0, 33, 0, 1, 500.0)])
     name, st, tiles = bench.dominant_fused(rec, {"fused": None})
     assert name == "k_rot_fused<12, 4>"
-    assert tiles == {11: 3, 12: 2}
+    assert tiles == {"k_rot_fused<11, 3>": 3, "k_rot_fused<12, 4>": 2}
     assert st["launches"] == 2 and st["ms"] == 80.0
     assert st["amps"] == 2 * 2.0 ** 33 and st["terms"] == 40
     rf = bench.roofline(name, st)
@@ -44,6 +47,14 @@ def test_dominant_fused_by_time_not_count():
     assert rf["algorithmic_bytes_per_launch"] == 32 * 2 ** 33
     assert rf["achieved"] == pytest.approx(32 * 2 ** 33 / 0.040 / 1e9)
     assert rf["frac"] == pytest.approx(rf["achieved"] / 8000.0)
+
+
+def test_half_tile_form_is_its_own_kernel():
+    rec = _recs([(2, 30, 12, 10, 30.0)] * 2 + [(2, 30, 12, 10, 25.0)] * 3)
+    rec["direct"][2:] = 7          # first, last and half-tile flags
+    name, st, tiles = bench.dominant_fused(rec, {"fused": None})
+    assert name == "k_rot_fused_half<12, 4>" and st["launches"] == 3
+    assert tiles == {"k_rot_fused<12, 4>": 2, "k_rot_fused_half<12, 4>": 3}
 
 
 def test_dominant_falls_back_without_records():
@@ -67,3 +78,15 @@ def test_more_than_one_gpu_exits_with_message():
                        capture_output=True, text=True, timeout=120)
     assert p.returncode != 0
     assert "one GPU" in (p.stdout + p.stderr)
+
+
+def test_batched_kernel_forms():
+    rec = _recs([(3, 20, 12, 64, 3.0)] * 2 + [(3, 20, 12, 64, 1.0)])
+    rec["direct"][:2] = 4
+    rec["amps"][:] = 3 << 30
+    name, st, kernels = bench.dominant_fused(rec, {"batched": None},
+                                             bench.REC_BATCH)
+    assert name == "k_rot_fused_batch<12, 4, true>"
+    assert st["amps"] == 2 * (3 << 30) and st["launches"] == 2
+    assert kernels == {"k_rot_fused_batch<12, 4, false>": 1,
+                       "k_rot_fused_batch<12, 4, true>": 2}

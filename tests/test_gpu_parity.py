@@ -306,8 +306,10 @@ def test_fused_direct_load_bit_identical(oracle, tile, low):
     """Tiles with at least 4 contiguous low bits are loaded straight into LDS
     (lanes permuted within 256-byte runs to match the LDS swizzle), the
     others through registers; with the direct form on, eligible first and
-    last micro-groups move their cosets between HBM and registers instead.
-    Every combination leaves the state bit-identical to the oracle."""
+    last micro-groups move their cosets between HBM and registers instead;
+    with the half-tile form on (tile 12), passes that have one run through
+    half a tile of LDS.  Every combination leaves the state bit-identical to
+    the oracle."""
     n = 18
     rng = np.random.default_rng(60 + tile + low)
     st = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
@@ -317,12 +319,54 @@ def test_fused_direct_load_bit_identical(oracle, tile, low):
     phi = 0.3 * hm.cf[perm] / np.max(np.abs(hm.cf))
     want = oracle.rotate(st, xm, zm, phi)
     for direct in (True, False):
-        try:
-            nat.set_fused_direct(direct)
-            got = nat.rotate(st, xm, zm, phi, nat.FUSED, tile, low)
-        finally:
-            nat.set_fused_direct(True)
-        assert np.array_equal(got, want), direct
+        for half in (0, nat.HALF_TILES):
+            try:
+                nat.set_fused_direct(direct)
+                nat.set_fused_half(half)
+                got = nat.rotate(st, xm, zm, phi, nat.FUSED, tile, low)
+            finally:
+                nat.set_fused_direct(True)
+                nat.set_fused_half(nat.HALF_TILES)
+            assert np.array_equal(got, want), (direct, half)
+
+
+@pytest.mark.parametrize("n,low,seed", [(16, 4, 1), (18, 4, 2), (18, 3, 3),
+                                        (20, 5, 4), (20, 2, 5)])
+def test_fused_half_tile_bit_identical(oracle, n, low, seed):
+    """Tile-12 passes in the half-tile form (exchanges between micro-groups
+    in two rounds through half a tile of LDS, split by a tile bit neither
+    group touches; first and last groups straight between HBM and
+    registers, whatever their lane layout) leave the state bit-identical to
+    the oracle, and the form is in fact taken (launch records, flag 4)."""
+    rng = np.random.default_rng(700 + seed)
+    st = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    hm = synth.hamiltonian(n, 1500, seed=seed, decay=True)
+    perm = oracle.sort_lex(hm.xm, hm.zm)
+    xm, zm = hm.xm[perm], hm.zm[perm]
+    phi = 0.3 * hm.cf[perm] / np.max(np.abs(hm.cf))
+    want = oracle.rotate(st, xm, zm, phi)
+    try:
+        nat.set_fused_half(1 << 12)
+        nat.prof_enable(True)
+        nat.prof_records(reset=True)
+        got = nat.rotate(st, xm, zm, phi, nat.FUSED, 12, low)
+        rec = nat.prof_records(reset=True)
+    finally:
+        nat.prof_enable(False)
+        nat.set_fused_half(nat.HALF_TILES)
+    fused = rec[rec["kind"] == 2]
+    half = fused[(fused["direct"] & 4) != 0]
+    multi = half[half["ngroups"] > 1]
+    print("half-tile passes", len(half), "of", len(fused), "multi-group",
+          len(multi), flush=True)
+    assert len(multi) > 0
+    assert np.array_equal(got, want)
+
+
+def test_fused_half_mask_rejects_other_tiles():
+    with pytest.raises(RuntimeError):
+        nat.set_fused_half(1 << 11)
+    nat.set_fused_half(nat.HALF_TILES)
 
 
 def _batch_regs(widths, depth=120, samples=2):
@@ -333,6 +377,33 @@ def _batch_regs(widths, depth=120, samples=2):
         regs.append(dict(hm=hm, det_idx=idx, det_cf=cf, samples=samples,
                          depth=depth, step_size=0.03, seed=0x9000 + i))
     return regs
+
+
+@pytest.mark.parametrize("half", [0, nat.HALF_TILES])
+def test_batch_qdrift_half_tile_vs_oracle(oracle, half):
+    """Batched tile-12 launches with and without the half-tile form (the
+    passes that have it in one launch, the others in a second per pass
+    index) give exactly each register's own qdrift overlaps."""
+    regs = _batch_regs([12, 14, 13, 15, 12, 16], depth=160)
+    try:
+        nat.set_fused_half(half)
+        nat.prof_enable(True)
+        nat.prof_records(reset=True)
+        got, st = nat.batch_qdrift(regs, nat.BATCH_FUSED_BATCHED, 12, 4)
+        rec = nat.prof_records(reset=True)
+    finally:
+        nat.prof_enable(False)
+        nat.set_fused_half(nat.HALF_TILES)
+    rec = rec[rec["kind"] == 3]
+    nhalf = int(np.sum((rec["direct"] & 4) != 0))
+    print("batched launches", len(rec), "half-tile", nhalf, flush=True)
+    assert (nhalf > 0) == bool(half)
+    for r, g in zip(regs, got):
+        want, _ = oracle.qdrift(r["hm"], dict(det_idx=r["det_idx"],
+                                              det_cf=r["det_cf"]),
+                                r["samples"], r["depth"], r["step_size"],
+                                r["seed"], want_state=False)
+        assert np.array_equal(g, want)
 
 
 @pytest.mark.parametrize("mode,tile", [(nat.BATCH_PLAIN_EACH, 12),
