@@ -652,6 +652,12 @@ def main(argv=None):
     schedule = args.schedule
     if schedule == "auto":
         schedule = "learned" if costmodel.PARAMS_PATH.is_file() else "greedy"
+        if schedule == "learned":
+            try:
+                costmodel.load()
+            except ValueError as e:   # trained for other features
+                log(f"bench: {e}; greedy schedule")
+                schedule = "greedy"
     if schedule == "learned":
         params, _ = costmodel.load()
         costmodel.install(params, min_qb=30)

@@ -12,6 +12,7 @@
  *   f[84]      tile bits / 13
  *   f[85]      first micro-group reads HBM directly (0/1)
  *   f[86]      last micro-group writes HBM directly (0/1)
+ *   f[87]      half-tile form (0/1)
  * as y = ln(ms) - (nqb - 30) ln 2 (time per 2^30 amplitudes).  For a block of
  * terms the scheduler enumerates passes [i, j) that end where the next term
  * would widen the tile (or at the longest feasible end), for tile widths 11
@@ -35,7 +36,7 @@ extern "C" {
 #endif
 
 #define PH2_LM_NQ 40
-#define PH2_LM_NIN (2 * PH2_LM_NQ + 7)
+#define PH2_LM_NIN (2 * PH2_LM_NQ + 8)
 
 /* Feature vector (PH2_LM_NIN floats) of the pass rot[0..n) in tiles of
  * tile_bits with the given shape (ph2hip_pass_shape, 4 entries). */

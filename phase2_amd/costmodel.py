@@ -1,6 +1,6 @@
 """Learned cost model of fused passes (include/phase2_learned.h).
 
-An FP32 perceptron (85 -> 64 -> 64 -> 1, ReLU) predicts the time of a fused
+An FP32 perceptron (88 -> 64 -> 64 -> 1, ReLU) predicts the time of a fused
 pass from the mask-bit features of its terms; the scheduler in
 csrc/sched_learned.c uses it to choose pass boundaries and tile widths.  The
 network is trained on the GPU by the library's own kernels (mlp.hip:
@@ -10,8 +10,9 @@ profiling (ph2hip_prof_records).
 Feature vector (must match ph2_learned_features): bits 0..39 of the OR of
 the pass's flip masks, bits 0..39 of the OR of its phase masks, then
 terms/48, micro-groups/48, gathered dims/13, contiguous low bits/13,
-tile bits/13, and whether the first / last micro-group moves its cosets
-straight between HBM and registers.  Target: ln(ms) - (nqb - 30) ln 2.
+tile bits/13, whether the first / last micro-group moves its cosets
+straight between HBM and registers, and whether the pass takes the
+half-tile form.  Target: ln(ms) - (nqb - 30) ln 2.
 """
 
 from __future__ import annotations
@@ -26,7 +27,7 @@ import numpy as np
 from . import _native as nat
 
 NQ = 40
-NIN = 2 * NQ + 7
+NIN = 2 * NQ + 8
 NHID = 64
 PARAMS_PATH = Path(__file__).resolve().parent / "costmodel_params.npz"
 LN2 = math.log(2.0)
@@ -148,6 +149,7 @@ def features_from_records(rec):
     X[:, 2 * NQ + 4] = rec["tile_bits"] / 13.0
     X[:, 2 * NQ + 5] = rec["direct"] & 1
     X[:, 2 * NQ + 6] = (rec["direct"] >> 1) & 1
+    X[:, 2 * NQ + 7] = (rec["direct"] >> 2) & 1
     y = (np.log(rec["ms"]) - (rec["nqb"].astype(np.float64) - 30) * LN2
          ).astype(np.float32)
     return X, y, rec
@@ -273,15 +275,30 @@ def save(params, path=PARAMS_PATH, **meta):
              **meta)
 
 
+def n_params(nin=NIN, nhid=NHID):
+    return nhid * nin + nhid + nhid * nhid + nhid + nhid + 1
+
+
 def load(path=PARAMS_PATH):
+    """Parameters and metadata of a saved network; ValueError if it was
+    trained for another feature vector."""
     d = np.load(path, allow_pickle=False)
-    return d["params"], {k: d[k] for k in d.files if k != "params"}
+    params = d["params"]
+    if int(d["nin"]) != NIN or int(d["nhid"]) != NHID or \
+            params.size != n_params():
+        raise ValueError(f"{path}: network {int(d['nin'])} -> "
+                         f"{int(d['nhid'])}, this library needs {NIN} -> "
+                         f"{NHID}")
+    return params, {k: d[k] for k in d.files if k != "params"}
 
 
 def install(params, low=3, min_qb=30):
     """Use the learned schedule for fused registers of >= min_qb qubits
     (passes with low .. low + 2 contiguous low bits)."""
     p = np.ascontiguousarray(params, np.float32)
+    if p.size != n_params():
+        raise ValueError(f"{p.size} parameters, the network has "
+                         f"{n_params()}")
     L = nat.lib()
     L.ph2_learned_install.restype = C.c_int
     if L.ph2_learned_install(_p(p, _f32p), C.c_uint32(NHID), C.c_uint32(low),

@@ -55,6 +55,7 @@ void ph2_learned_features(const struct ph2hip_rot *rot, size_t n,
 	f[2 * PH2_LM_NQ + 4] = (float)tile_bits / 13.0f;
 	f[2 * PH2_LM_NQ + 5] = (float)(shape[3] & 1u);
 	f[2 * PH2_LM_NQ + 6] = (float)(shape[3] >> 1 & 1u);
+	f[2 * PH2_LM_NQ + 7] = (float)(shape[3] >> 2 & 1u);
 }
 
 static uint64_t block_key(const struct ph2hip_rot *rot, size_t n,
