@@ -621,7 +621,7 @@ def main(argv=None):
                     help="registers of qdrift_batch")
     ap.add_argument("--tile", type=int, default=0,
                     help="fused tile bits (0: the library's automatic width)")
-    ap.add_argument("--low", type=int, default=4)
+    ap.add_argument("--low", type=int, default=3)
     ap.add_argument("--plain-terms", type=int, default=24)
     ap.add_argument("--schedule", choices=("auto", "greedy", "learned"),
                     default="auto",

@@ -125,7 +125,7 @@ def run_multidet(paulis: list[str], coeffs: list[float], delta: float,
     xm, zm = pauli_masks(paulis, nqb)
     phi = delta * np.asarray(coeffs, dtype=np.float64)
     _native.init()
-    reg = _native.Register(nqb, _native.FUSED, 0, 4)
+    reg = _native.Register(nqb, _native.FUSED, 0, 3)
     try:
         reg.prep_multidet(idx, cf)
         for k0 in range(0, len(phi), 1024):   # circ_cache-sized blocks
@@ -175,5 +175,5 @@ def qdrift_batch(registers: list[dict], mode: str = "batched"):
                          step_size=float(r["step_size"]),
                          seed=int(r.get("seed", 0))))
     _native.init()
-    out, _ = _native.batch_qdrift(regs, modes[mode], 0, 4)
+    out, _ = _native.batch_qdrift(regs, modes[mode], 0, 3)
     return out

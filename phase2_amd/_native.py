@@ -121,7 +121,7 @@ def init():
         _err("ph2_init")
 
 
-def set_default(mode=FUSED, tile_bits=12, low_bits=4):
+def set_default(mode=FUSED, tile_bits=12, low_bits=3):
     lib().ph2_set_default(int(mode), int(tile_bits), int(low_bits))
 
 
@@ -175,7 +175,7 @@ def cmpsit(hm, prep_kw, samples, steps, length, depth, angle_det,
     return out, st
 
 
-def rotate(state, xm, zm, phi, mode=FUSED, tile_bits=12, low_bits=4):
+def rotate(state, xm, zm, phi, mode=FUSED, tile_bits=12, low_bits=3):
     st = np.array(state, dtype=np.complex128, copy=True)
     nqb = int(st.shape[0]).bit_length() - 1
     xm = np.ascontiguousarray(xm, dtype=np.uint64)
@@ -267,7 +267,7 @@ def synchronize():
 class Register:
     """A device-resident register (struct qreg behind ph2_reg_*)."""
 
-    def __init__(self, nqb, mode=FUSED, tile_bits=12, low_bits=4):
+    def __init__(self, nqb, mode=FUSED, tile_bits=12, low_bits=3):
         L = lib()
         L.ph2_reg_create.restype = C.c_void_p
         for f in ("ph2_reg_prep_multidet", "ph2_reg_rotate",
@@ -280,7 +280,7 @@ class Register:
         if not self.h:
             _err(f"ph2_reg_create({nqb})")
 
-    def configure(self, mode, tile_bits=12, low_bits=4):
+    def configure(self, mode, tile_bits=12, low_bits=3):
         lib().ph2_reg_configure(C.c_void_p(self.h), C.c_int(mode),
                                 C.c_uint32(tile_bits), C.c_uint32(low_bits))
 
@@ -403,7 +403,7 @@ class BatchStats(C.Structure):
 BATCH_PLAIN_EACH, BATCH_FUSED_EACH, BATCH_FUSED_BATCHED = 0, 1, 2
 
 
-def batch_qdrift(regs, mode=BATCH_FUSED_BATCHED, tile_bits=0, low_bits=4):
+def batch_qdrift(regs, mode=BATCH_FUSED_BATCHED, tile_bits=0, low_bits=3):
     """regs: list of dicts with hm, det_idx, det_cf, samples, depth,
     step_size, seed.  Returns (list of per-register overlap arrays, stats)."""
     keep = []

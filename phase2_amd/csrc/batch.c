@@ -286,7 +286,7 @@ static int autotune(struct batch *B, int mode, uint32_t *tb, uint32_t *low)
 		minq = B->b[i].r->nqb < minq ? B->b[i].r->nqb : minq;
 	double best = 1e300;
 	*tb = 12;
-	*low = 4;
+	*low = 3;
 	for (size_t a = 0; a < sizeof TB / sizeof *TB; a++) {
 		if (TB[a] > minq)
 			continue;
@@ -335,7 +335,7 @@ int ph2_batch_qdrift(const struct ph2_batch_reg *regs, size_t nreg, int mode,
 		tune = now_s() - t0;
 	} else if (tile_bits == 0) {
 		tile_bits = 12;
-		low_bits = 4;
+		low_bits = 3;
 	}
 	if (ph2hip_synchronize() < 0)
 		rc = -1;
