@@ -100,12 +100,14 @@ int ph2hip_set_fused_direct(int on);
  * form where they have one (the exchanges between micro-groups run in two
  * rounds through half a tile of LDS, so twice the workgroups fit a CU;
  * first and last groups move their cosets between HBM and registers).
- * Only tile 12 has that form (PH2HIP_HALF_TILES); other bits are an error.
+ * Tiles 12 and 13 have that form (PH2HIP_HALF_TILES; 256- and 512-thread
+ * workgroups, 32 and 64 KiB of LDS); other bits are an error.  Batched
+ * launches use it at tile 12.
  * Default: the PHASE2_HIP_HALF environment variable, else
  * PH2HIP_HALF_TILES.  Every
  * setting gives the same state.  No counterpart in the reference (its
  * CUDA backend has no fused pass). */
-#define PH2HIP_HALF_TILES (1u << 12)
+#define PH2HIP_HALF_TILES ((1u << 12) | (1u << 13))
 int ph2hip_set_fused_half(uint32_t tile_mask);
 
 

@@ -368,7 +368,7 @@ def set_fused_direct(on: bool):
         _err("ph2hip_set_fused_direct")
 
 
-HALF_TILES = 1 << 12   # PH2HIP_HALF_TILES: the default half-tile mask
+HALF_TILES = (1 << 12) | (1 << 13)   # PH2HIP_HALF_TILES: default mask
 
 
 def set_fused_half(tile_mask: int):

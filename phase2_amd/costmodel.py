@@ -188,7 +188,7 @@ def _span_growth(xs, low):
     return out
 
 
-def random_schedule(xm, rng, tiles=(11, 12), lows=(3, 4, 5), maxlen=48):
+def random_schedule(xm, rng, tiles=(12, 13), lows=(3, 4, 5), maxlen=48):
     """Cut a block into passes of random feasible length, tile width and
     contiguous low bits."""
     lens, tbs, lbs = [], [], []

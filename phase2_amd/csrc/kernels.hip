@@ -654,12 +654,11 @@ __device__ __forceinline__ void half_offsets(const FusedGroup &G, uint32_t h,
 // between groups q and q+1 the cosets move through the buffer in two
 // rounds, one per half of the tile (split bit xh of group q; the threads
 // of one half are the waves whose coset-index bit xk is that half).  One
-// coset per thread.
+// coset per thread: 2^(TB - D) threads.
 template <int TB, int D>
 __device__ __forceinline__ void tile_groups_half(v2d *__restrict__ a,
 	v2d *buf, const FusedPass &P, uint64_t base, uint32_t tid)
 {
-	static_assert(((1u << TB) >> D) == kBlock, "one coset per thread");
 	const uint32_t ngroups = pass_ngroups(P);
 	const uint32_t L = pass_L(P), g = pass_g(P);
 	uint32_t bu = coset_base<TB, D>(tid, P.grp[0].jposw[0],
@@ -851,9 +850,10 @@ __global__ __launch_bounds__(kBlock) void k_rot_fused(
 }
 
 // Half-tile form of k_rot_fused (descriptor flag 4): half the LDS, so
-// twice the workgroups per CU.
+// twice the workgroups per CU.  One coset per thread: 256 threads at tile
+// 12, 512 at tile 13.
 template <int TB, int D>
-__global__ __launch_bounds__(kBlock) void k_rot_fused_half(
+__global__ __launch_bounds__((1 << TB) >> D) void k_rot_fused_half(
 	double2 *__restrict__ amp, const FusedPass P, uint64_t ntiles)
 {
 	__shared__ v2d buf[1u << (TB - 1)];
@@ -1241,11 +1241,13 @@ constexpr int group_depth(int tb)
 	return tb >= 12 ? 4 : tb - 8;
 }
 
-// Tile widths with a half-tile form (it needs one coset per thread,
-// 2^(tb - D) = 256).  Only tile 12 gains from it: there the whole-tile LDS
-// (64 KiB) allows two workgroups per CU and the half form four; narrower
-// tiles already reach the register-limited count.
+// Tile widths with a half-tile form (one coset per thread: 256 threads at
+// tile 12, 512 at tile 13).  Tile 12 gains from it because the whole-tile
+// LDS (64 KiB) allows two workgroups per CU and the half form four; tile 13
+// goes from one 256-thread workgroup per CU to two of 512.  Narrower tiles
+// already reach the register-limited count.  Batched launches: tile 12.
 constexpr uint32_t kHalfTiles = PH2HIP_HALF_TILES;
+constexpr uint32_t kBatchHalfTiles = 1u << 12;
 
 bool g_pipe_init = false;
 uint32_t g_pipe_mask = 0;   // bit TB set: persistent pipelined form
@@ -1288,8 +1290,9 @@ int launch_fused(double2 *a, uint32_t nqb, const FusedPass &P,
 			if (P.dflags & 4u) {
 				hipLaunchKernelGGL(
 					(k_rot_fused_half<TB, group_depth(TB)>),
-					dim3((uint32_t)gx, (uint32_t)gy), dim3(kBlock),
-					0, g_stream, a, P, ntiles);
+					dim3((uint32_t)gx, (uint32_t)gy),
+					dim3((1u << TB) >> group_depth(TB)), 0,
+					g_stream, a, P, ntiles);
 				HIPCHK(hipGetLastError());
 				return prof_end(stop);
 			}
@@ -1457,9 +1460,11 @@ int build_pass_mode(FusedPass &P, const struct ph2hip_rot *r, size_t len,
 				const uint32_t hi = fr & ~15u;
 				h = 31 - __builtin_clz(hi ? hi : fr);
 			}
+			// the two top coset-index bits select waves
+			const uint32_t ktop = tb - D - 1;
 			xh[q] = (uint8_t)h;
 			xk[q] = (uint8_t)(q == 0 || h == xh[q - 1] ?
-				(q ? xk[q - 1] : 7) : 13 - xk[q - 1]);
+				(q ? xk[q - 1] : ktop) : 2 * ktop - 1 - xk[q - 1]);
 		}
 	}
 	for (uint32_t gi = 0; gi < ng; gi++) {
@@ -1655,7 +1660,7 @@ int launch_batch(double2 *amp, const FusedPass *descs, const BatchItem *items,
 {
 	const uint64_t gx = ntiles < kMaxGrid ? ntiles : kMaxGrid;
 	const uint64_t gy = (ntiles + gx - 1) / gx;
-	if constexpr ((kHalfTiles >> TB & 1) != 0) {
+	if constexpr ((kBatchHalfTiles >> TB & 1) != 0) {
 		if (half) {
 			hipLaunchKernelGGL(
 				(k_rot_fused_batch<TB, group_depth(TB), true>),
@@ -2269,7 +2274,7 @@ int ph2hip_batch_plan_build(const struct ph2hip_batch_job *jobs, size_t njobs,
 			size_t k0 = 0;
 			for (size_t p = 0; p < jobs[j].npass && !rc; p++) {
 				rc = 1;
-				if (half_mask() >> tile_bits & 1)
+				if (((half_mask() & kBatchHalfTiles) >> tile_bits) & 1)
 					rc = build_pass_mode(pl->desc[d],
 						jobs[j].rot + k0, jobs[j].pass_len[p],
 						tile_bits, low_bits, k0, true);

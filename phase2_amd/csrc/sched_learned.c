@@ -33,7 +33,9 @@ static struct {
 	size_t cap;
 } LM;
 
-static const uint32_t TILES[] = { 11, 12 };
+/* candidate tile widths: both have the half-tile form (tile 11 was rarely
+ * chosen once tile 12 had it) */
+static const uint32_t TILES[] = { 12, 13 };
 #define NLOW 3	/* contiguous low bits low, low+1, low+2 */
 
 void ph2_learned_features(const struct ph2hip_rot *rot, size_t n,

@@ -330,14 +330,16 @@ def test_fused_direct_load_bit_identical(oracle, tile, low):
             assert np.array_equal(got, want), (direct, half)
 
 
-@pytest.mark.parametrize("n,low,seed", [(16, 4, 1), (18, 4, 2), (18, 3, 3),
-                                        (20, 5, 4), (20, 2, 5)])
-def test_fused_half_tile_bit_identical(oracle, n, low, seed):
-    """Tile-12 passes in the half-tile form (exchanges between micro-groups
-    in two rounds through half a tile of LDS, split by a tile bit neither
-    group touches; first and last groups straight between HBM and
-    registers, whatever their lane layout) leave the state bit-identical to
-    the oracle, and the form is in fact taken (launch records, flag 4)."""
+@pytest.mark.parametrize("n,tile,low,seed", [
+    (16, 12, 4, 1), (18, 12, 4, 2), (18, 12, 3, 3), (20, 12, 5, 4),
+    (20, 12, 2, 5), (17, 13, 4, 6), (19, 13, 3, 7), (21, 13, 5, 8)])
+def test_fused_half_tile_bit_identical(oracle, n, tile, low, seed):
+    """Tile-12 and tile-13 passes in the half-tile form (exchanges between
+    micro-groups in two rounds through half a tile of LDS, split by a tile
+    bit neither group touches; first and last groups straight between HBM
+    and registers, whatever their lane layout) leave the state
+    bit-identical to the oracle, and the form is in fact taken (launch
+    records, flag 4)."""
     rng = np.random.default_rng(700 + seed)
     st = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
     hm = synth.hamiltonian(n, 1500, seed=seed, decay=True)
@@ -346,10 +348,10 @@ def test_fused_half_tile_bit_identical(oracle, n, low, seed):
     phi = 0.3 * hm.cf[perm] / np.max(np.abs(hm.cf))
     want = oracle.rotate(st, xm, zm, phi)
     try:
-        nat.set_fused_half(1 << 12)
+        nat.set_fused_half(1 << tile)
         nat.prof_enable(True)
         nat.prof_records(reset=True)
-        got = nat.rotate(st, xm, zm, phi, nat.FUSED, 12, low)
+        got = nat.rotate(st, xm, zm, phi, nat.FUSED, tile, low)
         rec = nat.prof_records(reset=True)
     finally:
         nat.prof_enable(False)
@@ -367,6 +369,7 @@ def test_fused_half_mask_rejects_other_tiles():
     with pytest.raises(RuntimeError):
         nat.set_fused_half(1 << 11)
     nat.set_fused_half(nat.HALF_TILES)
+    assert nat.HALF_TILES == (1 << 12) | (1 << 13)
 
 
 def _batch_regs(widths, depth=120, samples=2):

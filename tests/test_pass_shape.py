@@ -38,16 +38,18 @@ def _passes(n, seed, tile, low, maxlen=24):
     return out
 
 
-@pytest.mark.parametrize("n,seed,low", [(20, 1, 4), (24, 2, 3), (28, 3, 5)])
-def test_half_form_keeps_shape(n, seed, low):
-    passes = _passes(n, seed, 12, low)
+@pytest.mark.parametrize("n,seed,low,tile", [(20, 1, 4, 12), (24, 2, 3, 12),
+                                             (28, 3, 5, 12), (22, 4, 4, 13),
+                                             (30, 5, 3, 13)])
+def test_half_form_keeps_shape(n, seed, low, tile):
+    passes = _passes(n, seed, tile, low)
     nhalf = nmulti = 0
     for xm, zm in passes:
         try:
             nat.set_fused_half(0)
-            rc0, s0 = _shape(xm, zm, 12, low)
-            nat.set_fused_half(1 << 12)
-            rc1, s1 = _shape(xm, zm, 12, low)
+            rc0, s0 = _shape(xm, zm, tile, low)
+            nat.set_fused_half(1 << tile)
+            rc1, s1 = _shape(xm, zm, tile, low)
         finally:
             nat.set_fused_half(nat.HALF_TILES)
         assert rc0 == 0 and rc1 == 0
@@ -61,10 +63,10 @@ def test_half_form_keeps_shape(n, seed, low):
     assert nmulti > 0 and nhalf >= nmulti // 2
 
 
-def test_half_form_only_at_tile_12():
+def test_half_form_only_at_tiles_12_and_13():
     xm, zm = _passes(20, 4, 11, 4)[0]
     try:
-        nat.set_fused_half(1 << 12)
+        nat.set_fused_half(nat.HALF_TILES)
         rc, s = _shape(xm, zm, 11, 4)
     finally:
         nat.set_fused_half(nat.HALF_TILES)
