@@ -102,7 +102,7 @@ int ph2hip_set_fused_direct(int on);
  * first and last groups move their cosets between HBM and registers).
  * Tiles 12 and 13 have that form (PH2HIP_HALF_TILES; 256- and 512-thread
  * workgroups, 32 and 64 KiB of LDS); other bits are an error.  Batched
- * launches use it at tile 12.
+ * launches use it too.
  * Default: the PHASE2_HIP_HALF environment variable, else
  * PH2HIP_HALF_TILES.  Every
  * setting gives the same state.  No counterpart in the reference (its

@@ -278,7 +278,7 @@ static int measure_round(struct batch *B, size_t s, double *out)
  * keep the fastest. */
 static int autotune(struct batch *B, int mode, uint32_t *tb, uint32_t *low)
 {
-	static const uint32_t TB[] = { 11, 12 };
+	static const uint32_t TB[] = { 11, 12, 13 };
 	static const uint32_t LOW[] = { 3, 4, 5 };
 	const size_t lim = B->n < TUNE_REGS ? B->n : TUNE_REGS;
 	uint32_t minq = 64;

@@ -875,9 +875,10 @@ struct BatchItem {
 };
 
 // HALF: every descriptor of the launch has the half-tile form (half the
-// LDS, as k_rot_fused_half).
+// LDS and one coset per thread, as k_rot_fused_half).
 template <int TB, int D, bool HALF = false>
-__global__ __launch_bounds__(kBlock) void k_rot_fused_batch(
+__global__ __launch_bounds__(HALF ? (1 << TB) >> D : kBlock)
+void k_rot_fused_batch(
 	double2 *__restrict__ amp, const FusedPass *__restrict__ descs,
 	const BatchItem *__restrict__ items, uint32_t nitems, uint64_t ntiles)
 {
@@ -1245,9 +1246,9 @@ constexpr int group_depth(int tb)
 // tile 12, 512 at tile 13).  Tile 12 gains from it because the whole-tile
 // LDS (64 KiB) allows two workgroups per CU and the half form four; tile 13
 // goes from one 256-thread workgroup per CU to two of 512.  Narrower tiles
-// already reach the register-limited count.  Batched launches: tile 12.
+// already reach the register-limited count.  Batched launches too.
 constexpr uint32_t kHalfTiles = PH2HIP_HALF_TILES;
-constexpr uint32_t kBatchHalfTiles = 1u << 12;
+constexpr uint32_t kBatchHalfTiles = (1u << 12) | (1u << 13);
 
 bool g_pipe_init = false;
 uint32_t g_pipe_mask = 0;   // bit TB set: persistent pipelined form
@@ -1664,8 +1665,9 @@ int launch_batch(double2 *amp, const FusedPass *descs, const BatchItem *items,
 		if (half) {
 			hipLaunchKernelGGL(
 				(k_rot_fused_batch<TB, group_depth(TB), true>),
-				dim3((uint32_t)gx, (uint32_t)gy), dim3(kBlock), 0,
-				g_stream, amp, descs, items, nitems, ntiles);
+				dim3((uint32_t)gx, (uint32_t)gy),
+				dim3((1u << TB) >> group_depth(TB)), 0, g_stream,
+				amp, descs, items, nitems, ntiles);
 			HIPCHK(hipGetLastError());
 			return 0;
 		}

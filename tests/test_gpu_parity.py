@@ -382,17 +382,20 @@ def _batch_regs(widths, depth=120, samples=2):
     return regs
 
 
-@pytest.mark.parametrize("half", [0, nat.HALF_TILES])
-def test_batch_qdrift_half_tile_vs_oracle(oracle, half):
-    """Batched tile-12 launches with and without the half-tile form (the
-    passes that have it in one launch, the others in a second per pass
-    index) give exactly each register's own qdrift overlaps."""
-    regs = _batch_regs([12, 14, 13, 15, 12, 16], depth=160)
+@pytest.mark.parametrize("tile,half", [(12, 0), (12, nat.HALF_TILES),
+                                       (13, nat.HALF_TILES)])
+def test_batch_qdrift_half_tile_vs_oracle(oracle, tile, half):
+    """Batched tile-12 and tile-13 launches with and without the half-tile
+    form (the passes that have it in one launch, the others in a second per
+    pass index) give exactly each register's own qdrift overlaps."""
+    widths = [12, 14, 13, 15, 12, 16] if tile == 12 else [13, 14, 15, 16,
+                                                          13, 14]
+    regs = _batch_regs(widths, depth=160)
     try:
         nat.set_fused_half(half)
         nat.prof_enable(True)
         nat.prof_records(reset=True)
-        got, st = nat.batch_qdrift(regs, nat.BATCH_FUSED_BATCHED, 12, 4)
+        got, st = nat.batch_qdrift(regs, nat.BATCH_FUSED_BATCHED, tile, 4)
         rec = nat.prof_records(reset=True)
     finally:
         nat.prof_enable(False)
