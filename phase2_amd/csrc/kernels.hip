@@ -691,7 +691,10 @@ __device__ __forceinline__ void tile_groups_half(v2d *__restrict__ a,
 		uint32_t sr = half_slot<TB>(bn, h) << 4;
 		__asm__("" : "+v"(sw), "+v"(sr));
 		for (uint32_t q = 0; q < 2; q++) {
-			__syncthreads();   // the buffer's previous readers are done
+			// the buffer's previous readers are done (it has none before
+			// the first exchange: one tile per workgroup)
+			if (gi > 0 || q > 0)
+				__syncthreads();
 			if (half == q) {
 #pragma unroll
 				for (int w = 0; w < (1 << D); w++) {
