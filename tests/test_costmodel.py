@@ -12,14 +12,19 @@ from phase2_amd import costmodel as cm
 from phase2_amd import synth
 
 
-def test_features_match_c():
+@pytest.mark.parametrize("direct", [2, 5, 7])
+def test_features_match_c(direct):
+    """The Python feature mirror equals ph2_learned_features, including the
+    direct first/last-group and half-tile flags (bits 0, 1, 2)."""
     rng = np.random.default_rng(0)
     xm = rng.integers(0, 1 << 33, size=7, dtype=np.uint64)
     zm = rng.integers(0, 1 << 33, size=7, dtype=np.uint64)
-    f = cm.c_features(xm, zm, 12, (6, 5, 3, 2))
+    f = cm.c_features(xm, zm, 12, (6, 5, 3, direct))
+    assert f[-3:].tolist() == [direct & 1, direct >> 1 & 1, direct >> 2 & 1]
     rec = np.zeros(1, dtype=nat.REC_DTYPE)
     rec["kind"], rec["nqb"], rec["tile_bits"], rec["low_bits"] = 2, 33, 12, 6
-    rec["gdim"], rec["nterms"], rec["ngroups"], rec["direct"] = 5, 7, 3, 2
+    rec["gdim"], rec["nterms"], rec["ngroups"] = 5, 7, 3
+    rec["direct"] = direct
     rec["xunion"] = np.bitwise_or.reduce(xm)
     rec["zunion"] = np.bitwise_or.reduce(zm)
     rec["ms"] = 1.0
