@@ -227,7 +227,7 @@ def auto_tile(tile, nqb):
     """The library's tile width for a requested one (0: automatic)."""
     if tile:
         return tile
-    return max(9, 13 if nqb >= 26 else 12 if nqb > 18 else nqb - 6)
+    return max(9, 13 if nqb >= 26 else 12 if nqb > 16 else nqb - 5)
 
 
 def fused_name(tile, batched=False, half=False):

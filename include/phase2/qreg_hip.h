@@ -14,8 +14,7 @@
  * Defaults come from PHASE2_HIP_MODE (plain|fused), PHASE2_HIP_TILE (tile
  * bits, 9..13) and PHASE2_HIP_LOW (contiguous low bits); otherwise fused,
  * automatic tile width and 3.  Tile width 0 (the automatic one) is 13 from
- * 26 qubits, 12 below, narrowed to nqb - 6 (at least 9) below 19 qubits so
- * that a pass keeps at least 64 tiles.  Registers narrower than 9 qubits always use the plain
+ * 26 qubits, 12 from 17, nqb - 5 (at least 9) below.  Registers narrower than 9 qubits always use the plain
  * sweep.
  */
 

@@ -49,13 +49,14 @@ static void *g_sched_ctx;
 
 /* Tile width when none is set: 13 from 26 qubits (the half-tile form at
  * tile 13 sweeps 26 qubits 4.7% faster than at tile 12, and a pass still
- * has 8192 tiles), 12 below, narrowed for small registers so that a pass
- * still has at least 64 tiles (workgroups) to spread over the device
- * (measured at 16 qubits: tile 10 sweeps 1.2x faster than tile 12). */
+ * has 8192 tiles), 12 from 17, nqb - 5 below.  Small registers are
+ * latency-bound (a few dozen workgroups per pass); at 16 qubits a whole
+ * Trotter run takes 0.476 ms per step at tile 11, 0.503 at 10, 0.532 at 9
+ * and 0.564 at 12. */
 static uint32_t auto_tile(uint32_t nqb)
 {
-	const uint32_t t = nqb >= 26 ? 13 : nqb > 18 ? 12 :
-		(nqb > 6 ? nqb - 6 : 0);
+	const uint32_t t = nqb >= 26 ? 13 : nqb > 16 ? 12 :
+		(nqb > 5 ? nqb - 5 : 0);
 	return t < PH2HIP_TILE_MIN ? PH2HIP_TILE_MIN : t;
 }
 
