@@ -275,6 +275,55 @@ def test_fused_equals_plain_n24():
     assert np.max(np.abs(back - st)) < 1e-12
 
 
+def test_fused_undo_n33():
+    """At the headline width (33 qubits, amplitude indices beyond 2^32, the
+    128 GiB register) one 1024-term block of the lexicographic sweep under
+    the learned schedule (tile 12 and 13, half-tile and whole-tile passes),
+    then the same block reversed with negated angles, returns the
+    64-determinant state: its overlap with itself stays 1 to 1e-12 and the
+    norm to 1e-12.  An indexing error anywhere in the passes would move the
+    state far from that."""
+    from phase2_amd import costmodel as cm
+    w = synth.trott_n33()
+    hm = w["hm"]
+    perm = nat.sort_lex(hm.xm, hm.zm)
+    k0 = 42 * 1024
+    xm = hm.xm[perm][k0:k0 + 1024]
+    zm = hm.zm[perm][k0:k0 + 1024]
+    cfb = hm.cf[perm][k0:k0 + 1024]
+    phi = 0.3 * cfb / np.max(np.abs(cfb))
+    cf = w["det_cf"] / np.linalg.norm(w["det_cf"])
+    params, _ = cm.load()
+    nat.init()
+    reg = nat.Register(33, nat.FUSED, 0, 3)
+    try:
+        cm.install(params, min_qb=30)
+        nat.prof_enable(True)
+        nat.prof_records(reset=True)
+        reg.prep_multidet(w["det_idx"], cf)
+        print("n=33 block forward", flush=True)
+        reg.rotate(xm, zm, phi)
+        mid = reg.overlap(w["det_idx"], cf)
+        print("n=33 block backward", flush=True)
+        reg.rotate(xm[::-1], zm[::-1], -phi[::-1])
+        ov = reg.overlap(w["det_idx"], cf)
+        n2 = reg.norm2()
+        rec = nat.prof_records(reset=True)
+    finally:
+        nat.prof_enable(False)
+        cm.uninstall()
+        reg.free()
+    fused = rec[rec["kind"] == 2]
+    forms = {(int(t), bool(d & 4)) for t, d in zip(fused["tile_bits"],
+                                                  fused["direct"])}
+    print("n=33 overlap after the block", mid, "after undoing it", ov,
+          "norm2", n2, "pass forms", sorted(forms), flush=True)
+    assert abs(mid - 1.0) > 1e-6                 # the block did something
+    assert abs(ov - 1.0) < 1e-12
+    assert abs(n2 - 1.0) < 1e-12
+    assert (13, True) in forms
+
+
 @pytest.mark.parametrize("pipe_tile", [10, 11, 12])
 def test_fused_pipelined_form_bit_identical(oracle, pipe_tile):
     """The persistent pipelined fused kernel and every plain-kernel variant
