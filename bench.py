@@ -646,6 +646,9 @@ def main(argv=None):
     if args.steps is None:
         args.steps = DEFAULT_STEPS[args.workload]
 
+    # the library logs INFO lines to stdout (as the reference does); keep
+    # stdout to the one JSON line
+    os.environ.setdefault("PHASE2_LOG", "WARN")
     from phase2_amd import _native as nat
     from phase2_amd import costmodel
     nat.init()
