@@ -659,6 +659,8 @@ template <int TB, int D>
 __device__ __forceinline__ void tile_groups_half(v2d *__restrict__ a,
 	v2d *buf, const FusedPass &P, uint64_t base, uint32_t tid)
 {
+	// the split index bits (the top two) must select whole 64-lane waves
+	static_assert(TB - D >= 8 && TB - D <= 10, "2^(TB-D) threads, 256..1024");
 	const uint32_t ngroups = pass_ngroups(P);
 	const uint32_t L = pass_L(P), g = pass_g(P);
 	uint32_t bu = coset_base<TB, D>(tid, P.grp[0].jposw[0],
