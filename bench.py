@@ -663,7 +663,7 @@ def main(argv=None):
                 schedule = "greedy"
     if schedule == "learned":
         params, _ = costmodel.load()
-        costmodel.install(params, min_qb=30)
+        costmodel.install(params, min_qb=26)
 
     r = BENCH[args.workload](args, nat, costmodel, schedule)
     if schedule == "learned":

@@ -292,7 +292,7 @@ def load(path=PARAMS_PATH):
     return params, {k: d[k] for k in d.files if k != "params"}
 
 
-def install(params, low=3, min_qb=30):
+def install(params, low=3, min_qb=26):
     """Use the learned schedule for fused registers of >= min_qb qubits
     (passes with low .. low + 2 contiguous low bits)."""
     p = np.ascontiguousarray(params, np.float32)
