@@ -194,6 +194,12 @@ int ph2_learned_schedule(const struct ph2hip_rot *rot, size_t n, uint32_t nqb,
 		if (ph2hip_pass_shape(rot + cd.i, cd.j - cd.i, cd.tb, cd.low,
 			    shape) < 0)
 			continue;
+		/* a multi-group tile-13 pass without the half-tile form runs
+		 * one 128 KiB workgroup per CU; random cuts (the training data)
+		 * never produce one, so its prediction is a guess: leave it out
+		 * (every start keeps its tile-12 candidates) */
+		if (cd.tb == 13 && shape[2] > 1 && !(shape[3] & 4u))
+			continue;
 		LM.cand[k] = cd;
 		ph2_learned_features(rot + cd.i, cd.j - cd.i, cd.tb, shape,
 			LM.feat + k * PH2_LM_NIN);
