@@ -173,3 +173,17 @@ def test_learned_schedule_gain_over_greedy():
           f"learned {best['learned']:.4f} "
           f"gain {best['greedy'] / best['learned']:.4f}x")
     assert best["learned"] <= 1.10 * best["greedy"]
+
+
+def test_load_rejects_other_feature_vectors(tmp_path):
+    """A parameter file trained for another feature vector (or a truncated
+    array) is refused before it can reach ph2_learned_install."""
+    p, meta = cm.load()
+    assert p.size == cm.n_params() and int(meta["nin"]) == cm.NIN
+    bad = tmp_path / "old.npz"
+    np.savez(bad, params=np.zeros(cm.n_params(cm.NIN - 1), np.float32),
+             nin=cm.NIN - 1, nhid=cm.NHID)
+    with pytest.raises(ValueError):
+        cm.load(bad)
+    with pytest.raises(ValueError):
+        cm.install(p[:-1])
