@@ -115,9 +115,20 @@ int ph2hip_set_fused_half(uint32_t tile_mask);
  * entries): shape[0] = contiguous low bits L, shape[1] = gathered
  * dimensions g, shape[2] = micro-groups, shape[3] = which of the first
  * (bit 0) and last (bit 1) micro-groups move their cosets straight between
- * HBM and registers.  -1 if the terms do not fit one tile. */
+ * HBM and registers, and bit 2 for the half-tile form.  -1 if the terms do
+ * not fit one tile. */
 int ph2hip_pass_shape(const struct ph2hip_rot *rot, size_t n,
 	uint32_t tile_bits, uint32_t low_bits, uint32_t *shape);
+
+/* Micro-groups of that pass (for inspection and host-side tests): per group
+ * PH2HIP_GROUP_WORDS words, the basis vectors u[0..3] (tile coordinates),
+ * the tile bit of each coset-index bit jpos[0..11], the exchange split bit
+ * xh and coset-index bit xk towards the next group (half-tile form), the
+ * first term and the term count.  Returns the number of groups, -1 if the
+ * terms do not fit one tile or cap (in groups) is too small. */
+#define PH2HIP_GROUP_WORDS 20
+long ph2hip_pass_groups(const struct ph2hip_rot *rot, size_t n,
+	uint32_t tile_bits, uint32_t low_bits, uint32_t *out, size_t cap);
 
 /* Dimension of span{x_k & ~(2^low_bits - 1)} over GF(2) for k in [0, n):
  * the number of tile bits beyond low_bits a pass over these terms needs. */

@@ -2499,6 +2499,33 @@ int ph2hip_gather_abs(const void *amp, uint64_t namp_total,
 	return ph2hip_gather(amp, bits, idx, n, out);
 }
 
+long ph2hip_pass_groups(const struct ph2hip_rot *rot, size_t n,
+	uint32_t tile_bits, uint32_t low_bits, uint32_t *out, size_t cap)
+{
+	if (n == 0 || n > PH2HIP_PASS_MAX || tile_bits < PH2HIP_TILE_MIN ||
+	    tile_bits > PH2HIP_TILE_MAX)
+		return -1;
+	FusedPass P;
+	if (build_pass(P, rot, n, tile_bits, low_bits, 0))
+		return -1;
+	if (P.ngroups > cap)
+		return fail("pass_groups: %u groups, room for %zu", P.ngroups,
+			cap);
+	for (uint32_t gi = 0; gi < P.ngroups; gi++) {
+		const FusedGroup &G = P.grp[gi];
+		uint32_t *o = out + (size_t)gi * PH2HIP_GROUP_WORDS;
+		for (int i = 0; i < 4; i++)
+			o[i] = G.u[i];
+		for (int i = 0; i < 12; i++)
+			o[4 + i] = G.jpos[i];
+		o[16] = G.xh;
+		o[17] = G.xk;
+		o[18] = G.first;
+		o[19] = G.count;
+	}
+	return P.ngroups;
+}
+
 int ph2hip_pass_shape(const struct ph2hip_rot *rot, size_t n,
 	uint32_t tile_bits, uint32_t low_bits, uint32_t *shape)
 {

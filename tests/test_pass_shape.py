@@ -73,3 +73,84 @@ def test_half_form_only_at_tiles_12_and_13():
     assert rc == 0 and not s[3] & 4
     with pytest.raises(RuntimeError):
         nat.set_fused_half(1 << 11)
+
+
+# -- the half-tile exchange, emulated on the host ----------------------------
+
+def _groups(xm, zm, tile, low):
+    rot = (nat.Rot * len(xm))()
+    for k in range(len(xm)):
+        rot[k] = nat.Rot(int(xm[k]), int(zm[k]), 1.0, 0.0)
+    out = (C.c_uint32 * (20 * 48))()
+    L = nat.lib()
+    L.ph2hip_pass_groups.restype = C.c_long
+    ng = L.ph2hip_pass_groups(rot, C.c_size_t(len(xm)), C.c_uint32(tile),
+                              C.c_uint32(low), out, C.c_size_t(48))
+    assert ng > 0
+    a = np.frombuffer(out, dtype=np.uint32)[:20 * ng].reshape(ng, 20)
+    return [dict(u=[int(x) for x in r[0:4]], jpos=[int(x) for x in r[4:16]],
+                 xh=int(r[16]), xk=int(r[17]), first=int(r[18]),
+                 count=int(r[19])) for r in a]
+
+
+def _lds_slot(v):
+    return v ^ (((v >> 4) ^ (v >> 8)) & 15)
+
+
+def _half_slot(v, h, tb):
+    # the kernel's map: bits h and tb-1 exchanged, slot map, bit tb-1 dropped
+    t = ((v >> h) ^ (v >> (tb - 1))) & 1
+    v ^= t * ((1 << h) | (1 << (tb - 1)))
+    return _lds_slot(v) & ((1 << (tb - 1)) - 1)
+
+
+def _coset(g, j, w, tb, d):
+    bu = 0
+    for k in range(tb - d):
+        bu |= ((j >> k) & 1) << g["jpos"][k]
+    for i in range(d):
+        if w >> i & 1:
+            bu ^= g["u"][i]
+    return bu
+
+
+@pytest.mark.parametrize("n,seed,low,tile", [(20, 11, 3, 12), (24, 12, 4, 12),
+                                             (22, 13, 3, 13), (30, 14, 5, 13)])
+def test_half_tile_exchange_moves_every_amplitude(n, seed, low, tile):
+    """For every exchange of a half-tile pass: each thread's coset lies in
+    the half of the tile its wave handles, the write slots of group q fill
+    each half exactly once, and group q+1 reads back from each slot the
+    very tile coordinate it expects (the kernel's slot map, restated)."""
+    d = 4
+    nthreads = 1 << (tile - d)
+    checked = 0
+    for xm, zm in _passes(n, seed, tile, low, maxlen=40):
+        rc, sh = _shape(xm, zm, tile, low)
+        assert rc == 0
+        if not sh[3] & 4 or sh[2] < 2:
+            continue
+        gs = _groups(xm, zm, tile, low)
+        assert len(gs) == sh[2]
+        assert sum(g["count"] for g in gs) == len(xm)
+        for q in range(len(gs) - 1):
+            g0, g1 = gs[q], gs[q + 1]
+            h, xk = g0["xh"], g0["xk"]
+            assert xk in (tile - d - 1, tile - d - 2)   # wave-select bits
+            assert g0["jpos"][xk] == h and g1["jpos"][xk] == h
+            table = {}
+            for j in range(nthreads):
+                half = (j >> xk) & 1
+                for w in range(1 << d):
+                    v = _coset(g0, j, w, tile, d)
+                    assert (v >> h) & 1 == half
+                    key = (half, _half_slot(v, h, tile))
+                    assert key not in table
+                    table[key] = v
+            assert len(table) == 1 << tile
+            for j in range(nthreads):
+                half = (j >> xk) & 1
+                for w in range(1 << d):
+                    v = _coset(g1, j, w, tile, d)
+                    assert table[(half, _half_slot(v, h, tile))] == v
+            checked += 1
+    assert checked > 0
