@@ -125,7 +125,9 @@ static size_t candidates(const struct ph2hip_rot *rot, size_t n,
 			const size_t t = c / NLOW;
 			const uint32_t dl = (uint32_t)(c % NLOW);
 			const uint32_t tb = TILES[t] < nqb ? TILES[t] : nqb;
-			if (tb < PH2HIP_TILE_MIN || (t > 0 && tb == TILES[t - 1]))
+			/* narrow registers clamp both widths to nqb: once */
+			if (tb < PH2HIP_TILE_MIN || (t > 0 && tb ==
+				    (TILES[t - 1] < nqb ? TILES[t - 1] : nqb)))
 				continue;
 			const uint32_t low = LM.low + dl < tb ? LM.low + dl : tb;
 			if (dl > 0 && low == (LM.low + dl - 1 < tb ?
