@@ -1450,8 +1450,9 @@ int build_pass_mode(FusedPass &P, const struct ph2hip_rot *r, size_t len,
 	// Half-tile exchanges: between groups q and q+1 the tile moves through
 	// LDS in two halves split by tile bit xh[q], which no vector of either
 	// span touches, so that every coset lies in one half; coset-index bit
-	// xk[q] (a wave-select bit, 6 or 7) maps to xh[q] in both groups, so
-	// that a thread writes and reads in the same half.
+	// xk[q] (one of the top two, which select whole waves: 6/7 at tile 12,
+	// 7/8 at tile 13) maps to xh[q] in both groups, so that a thread
+	// writes and reads in the same half.
 	uint8_t xh[PH2HIP_PASS_MAX] = {}, xk[PH2HIP_PASS_MAX] = {};
 	if (half) {
 		for (uint32_t q = 0; q + 1 < ng; q++) {
