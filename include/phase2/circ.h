@@ -6,9 +6,11 @@
  * cache and per-step results (replaces include/phase2/circ.h:47-299 of the
  * reference; same types, names, ownership and return conventions).
  *
- * circ_init adopts hm and the state-prep payload by value and frees them
- * itself on error; circ_free releases everything; the *_free helpers are
- * idempotent.  Functions return 0 or -1 (with log_error).
+ * Every struct has the reference's members in the reference's order (the
+ * same layout and size).  circ_init adopts hm and the state-prep payload
+ * by value and frees them itself on error; circ_free releases everything;
+ * the *_free helpers are idempotent.  Functions return 0 or -1 (with
+ * log_error).
  */
 
 #include <stddef.h>
@@ -69,7 +71,6 @@ struct data_coeff_matrix {
 };
 
 struct circ_cache;
-struct circ_measure_plan;
 
 struct circ {
 	struct circ_hamil hm;
@@ -80,8 +81,6 @@ struct circ {
 	enum stprep_kind stprep_kind;
 	struct data_coeff_matrix cm;
 	struct state_prep_coeff_scratch sp_scratch;
-	/* Device-side overlap plan built by circ_prepst (private). */
-	struct circ_measure_plan *mplan;
 };
 
 int circ_hamil_init(struct circ_hamil *hm, uint32_t qb, size_t len);

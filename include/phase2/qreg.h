@@ -7,6 +7,9 @@
  * 2^qb complex128 amplitudes (interleaved re, im; 16 B each), allocated
  * with hipMalloc; there is no partner-exchange buffer (qb_hi = 0, buf NULL).
  *
+ * The struct has the reference's members in the reference's order, so code
+ * compiled against either header sees the same layout.
+ *
  * amp is a host shadow of the device state, used only by the bulk
  * qreg_sync_* transfers.  It is allocated at qreg_init for registers of at
  * most QREG_SHADOW_MAX_QB qubits and is NULL above that, so a 33-qubit
@@ -33,6 +36,15 @@ struct qreg {
 
 	_Complex double *amp, *buf;
 	uint64_t namp;
+
+	/* The reference's MPI exchange bookkeeping (msg_count, the request
+	 * arrays, nreqs).  One GPU exchanges nothing: always 0 / NULL.  Kept
+	 * so that the struct -- and every struct that embeds it, struct circ
+	 * and the algorithm drivers -- has the reference's layout and size
+	 * (the request arrays are MPI_Request pointers there). */
+	int msg_count;
+	void *reqs_snd, *reqs_rcv;
+	size_t nreqs;
 
 	/* Backend handle (struct qreg_hip, private to qreg_hip.c). */
 	void *backend;

@@ -31,13 +31,15 @@ extern "C" {
 
 struct data_coeff_matrix;
 
+/* The reference's layout.  tup_a points into a block that
+ * state_prep_coeff_scratch_init allocates with the library's private
+ * tables in front of it (occupation masks, device mirror); only
+ * state_prep_coeff_scratch_free releases it. */
 struct state_prep_coeff_scratch {
 	uint32_t n_sites, n_alpha, n_beta;
 	uint32_t *tup_a, *tup_b;	/* Ma x max(n_alpha,1), Mb x max(n_beta,1) */
 	double *det_a, *det_b;		/* host determinants of the last block */
 	size_t Ma, Mb;
-	uint64_t *mask_a, *mask_b;	/* occupation bit masks, host */
-	void *dev;			/* device mirror (private) */
 };
 
 int state_prep_coeff_scratch_init(struct state_prep_coeff_scratch *sc,

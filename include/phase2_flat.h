@@ -100,6 +100,12 @@ int ph2_reg_overlap_multidet(void *reg, size_t ndet, const uint64_t *didx,
 	const double *dcf, double *out);
 int ph2_reg_norm2(void *reg, double *out);
 int ph2_reg_read(void *reg, double *state);
+/* out[2k], out[2k+1] = amp[idx[k]] (indices below 2^nqb). */
+int ph2_reg_gather(void *reg, size_t n, const uint64_t *idx, double *out);
+/* The reproducible dense state of ph2hip_fill_hash, and the count of
+ * components in which two registers of one width differ. */
+int ph2_reg_fill_hash(void *reg, uint64_t seed);
+int ph2_reg_count_unequal(void *reg1, void *reg2, uint64_t *count);
 
 #ifdef __cplusplus
 }

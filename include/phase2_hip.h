@@ -258,6 +258,17 @@ int ph2hip_elapsed_ms(int slot_start, int slot_stop, double *ms);
 /* sum_i |amp[i]|^2 (norm-drift reporting), fixed reduction order. */
 int ph2hip_norm2(const void *amp, uint32_t nqb, double *out);
 
+/* Checks at full register width without host copies (no counterpart in the
+ * reference).  fill_hash writes a reproducible dense state: with
+ * g = 0x9e3779b97f4a7c15 and m the splitmix64 output mix,
+ *   amp[i] = (u(m(seed + (2i+1) g)), u(m(seed + (2i+2) g))),
+ *   u(h) = (h >> 11) * 2^-52 - 1  (exact, in [-1, 1)).
+ * count_unequal counts the components (re and im separately) in which two
+ * registers differ as IEEE doubles (-0 == +0); it waits for the result. */
+int ph2hip_fill_hash(void *amp, uint64_t namp, uint64_t seed);
+int ph2hip_count_unequal(const void *a, const void *b, uint64_t namp,
+	uint64_t *count);
+
 /* -- cost-model network ----------------------------------------------------- */
 
 /* FP32 perceptron nin -> nhid -> nhid -> 1 with ReLU (mlp.hip): forward,

@@ -57,15 +57,18 @@ def run(paulis: list[str], coeffs: list[float], delta: float,
 def pauli_masks(paulis: list[str], nqb: int):
     """Flip and phase masks of Pauli strings in "X0 Y1 Z5" notation, read
     as ``run`` reads them (phase2/phase2_run.c:60-103): tokens separated by
-    blanks, tabs or newlines, each an operator in IXYZ, an optional sign and
-    decimal digits, the qubit below nqb ("-0" is qubit 0); a later operator
-    on the same qubit replaces the earlier one.  Raises ValueError
-    otherwise."""
+    blanks, tabs or newlines, each an operator in IXYZ followed by the qubit
+    field as C's strtoul reads it (leading \\v, \\f or \\r skipped, an
+    optional sign, decimal digits, the value modulo 2^64 and negated as an
+    unsigned number after '-', a magnitude of 2^64 or more saturating), the
+    qubit below nqb; a later operator on the same qubit replaces the earlier
+    one.  Raises ValueError otherwise."""
     import re
     import numpy as np
     xm = np.zeros(len(paulis), dtype=np.uint64)
     zm = np.zeros(len(paulis), dtype=np.uint64)
-    tok_re = re.compile(r"([IXYZ])([+-]?)([0-9]+)")
+    tok_re = re.compile(r"([IXYZ])[\v\f\r]*([+-]?)([0-9]+)")
+    top = 1 << 64
     for k, s in enumerate(paulis):
         if len(s) >= 4096:
             raise ValueError("Pauli string of 4096 characters or more")
@@ -74,10 +77,14 @@ def pauli_masks(paulis: list[str], nqb: int):
             if not tok:
                 continue
             m = tok_re.fullmatch(tok)
-            q = int(m.group(3)) if m else -1
-            if m and m.group(2) == "-" and q != 0:
-                q = -1   # wraps to a huge qubit index
-            if not m or q < 0 or q >= nqb:
+            q = -1
+            if m:
+                q = int(m.group(3))
+                if q >= top:
+                    q = top - 1          # ERANGE: ULONG_MAX
+                elif m.group(2) == "-":
+                    q = (top - q) % top  # unsigned negation
+            if not m or q >= nqb:
                 raise ValueError(f"bad Pauli token {tok!r} in {s!r}")
             b = 1 << q
             x &= ~b

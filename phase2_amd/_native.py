@@ -264,6 +264,26 @@ def synchronize():
         _err("ph2hip_synchronize")
 
 
+def hash_state(nqb, seed):
+    """The state ph2hip_fill_hash writes, restated in numpy (uint64
+    arithmetic wraps as in the kernel)."""
+    g = np.uint64(0x9e3779b97f4a7c15)
+    i = np.arange(1 << nqb, dtype=np.uint64)
+
+    def mix(x):
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xbf58476d1ce4e5b9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94d049bb133111eb)
+        return x ^ (x >> np.uint64(31))
+
+    def unit(h):
+        return (h >> np.uint64(11)).astype(np.float64) * 2.0 ** -52 - 1.0
+
+    with np.errstate(over="ignore"):
+        re = unit(mix(np.uint64(seed) + (np.uint64(2) * i + np.uint64(1)) * g))
+        im = unit(mix(np.uint64(seed) + (np.uint64(2) * i + np.uint64(2)) * g))
+    return re + 1j * im
+
+
 class Register:
     """A device-resident register (struct qreg behind ph2_reg_*)."""
 
@@ -323,6 +343,31 @@ class Register:
         if lib().ph2_reg_read(C.c_void_p(self.h), _p(st, _f64p)) != 0:
             _err("ph2_reg_read")
         return st
+
+    def gather(self, idx):
+        idx = np.ascontiguousarray(idx, dtype=np.uint64)
+        out = np.zeros(idx.shape[0], dtype=np.complex128)
+        L = lib()
+        L.ph2_reg_gather.restype = C.c_int
+        if L.ph2_reg_gather(C.c_void_p(self.h), C.c_size_t(idx.shape[0]),
+                            _p(idx, _u64p), _p(out, _f64p)) != 0:
+            _err("ph2_reg_gather")
+        return out
+
+    def fill_hash(self, seed):
+        L = lib()
+        L.ph2_reg_fill_hash.restype = C.c_int
+        if L.ph2_reg_fill_hash(C.c_void_p(self.h), C.c_uint64(seed)) != 0:
+            _err("ph2_reg_fill_hash")
+
+    def count_unequal(self, other):
+        n = C.c_uint64(0)
+        L = lib()
+        L.ph2_reg_count_unequal.restype = C.c_int
+        if L.ph2_reg_count_unequal(C.c_void_p(self.h), C.c_void_p(other.h),
+                                   C.byref(n)) != 0:
+            _err("ph2_reg_count_unequal")
+        return int(n.value)
 
     def free(self):
         if self.h:

@@ -152,8 +152,9 @@ static void free_payload(struct circ *ct)
 	}
 }
 
-static void plan_free(struct circ_measure_plan *p)
+static void plan_free(void *v)
 {
+	struct circ_measure_plan *p = v;
 	if (!p)
 		return;
 	free(p->idx);
@@ -168,7 +169,6 @@ int circ_init(struct circ *ct, struct circ_hamil hm, enum stprep_kind sp_kind,
 	memset(&ct->cm, 0, sizeof ct->cm);
 	memset(&ct->sp_scratch, 0, sizeof ct->sp_scratch);
 	ct->cache = NULL;
-	ct->mplan = NULL;
 	ct->hm = hm;
 	ct->stprep_kind = sp_kind;
 
@@ -224,14 +224,22 @@ void circ_free(struct circ *ct)
 	free_payload(ct);
 	circ_cache_free(ct->cache);
 	ct->cache = NULL;
-	plan_free(ct->mplan);
-	ct->mplan = NULL;
-	qreg_free(&ct->reg);
+	qreg_free(&ct->reg);	/* with it the overlap plan */
 }
 
 int circ_prepst(struct circ *ct)
 {
 	if (ct->stprep_kind == STPREP_MULTIDET) {
+		/* an index beyond the register aliases silently in the
+		 * reference (phase2/qreg.c:35-47); here it is an error */
+		for (size_t d = 0; d < ct->md.len; d++)
+			if (ct->md.dets[d].idx >= ct->reg.namp) {
+				log_error("circ_prepst: determinant %zu index "
+					  "%llu beyond %u qubits", d,
+					(unsigned long long)ct->md.dets[d].idx,
+					ct->reg.qb_lo);
+				return -1;
+			}
 		qreg_zero(&ct->reg);
 		for (size_t d = 0; d < ct->md.len; d++)
 			qreg_setamp(&ct->reg, ct->md.dets[d].idx,
@@ -280,24 +288,33 @@ int circ_step_reverse(struct circ *ct, const struct circ_hamil *hm,
 	return step_generic(ct, hm, omega, true);
 }
 
+/* The gather plan of the multidet overlap, built on the first measure and
+ * kept with the register. */
 static struct circ_measure_plan *plan_multidet(struct circ *ct)
 {
-	if (ct->mplan)
-		return ct->mplan;
-	struct circ_measure_plan *p = calloc(1, sizeof *p);
+	struct circ_measure_plan *p = qreg_hip_aux(&ct->reg);
+	if (p)
+		return p;
 	const size_t n = ct->md.len;
+	for (size_t d = 0; d < n; d++)
+		if (ct->md.dets[d].idx >= ct->reg.namp) {
+			log_error("circ_measure: determinant %zu beyond the "
+				  "register", d);
+			return NULL;
+		}
+	p = calloc(1, sizeof *p);
 	if (!p)
 		return NULL;
 	p->idx = malloc(sizeof *p->idx * (n ? n : 1));
 	p->amp = malloc(sizeof *p->amp * 2 * (n ? n : 1));
-	if (!p->idx || !p->amp) {
+	if (!p->idx || !p->amp || qreg_hip_set_aux(&ct->reg, p,
+		    plan_free) < 0) {
 		plan_free(p);
 		return NULL;
 	}
 	for (size_t d = 0; d < n; d++)
-		p->idx[d] = ct->md.dets[d].idx & (ct->reg.namp - 1);
+		p->idx[d] = ct->md.dets[d].idx;
 	p->n = n;
-	ct->mplan = p;
 	return p;
 }
 
@@ -306,7 +323,7 @@ _Complex double circ_measure(struct circ *ct)
 	if (ct->stprep_kind == STPREP_MULTIDET) {
 		struct circ_measure_plan *p = plan_multidet(ct);
 		if (!p) {
-			log_error("circ_measure: out of memory");
+			log_error("circ_measure: no overlap plan");
 			return 0.0;
 		}
 		if (ph2hip_gather(qreg_hip_device_ptr(&ct->reg),

@@ -383,8 +383,9 @@ int ph2_reg_overlap_multidet(void *r, size_t ndet, const uint64_t *didx,
 	double *a = malloc(sizeof *a * 2 * (ndet ? ndet : 1));
 	int rc = -1;
 	if (idx && a) {
+		/* indices beyond the register are refused by the gather */
 		for (size_t d = 0; d < ndet; d++)
-			idx[d] = didx[d] & (reg->namp - 1);
+			idx[d] = didx[d];
 		rc = ph2hip_gather(qreg_hip_device_ptr(reg), reg->qb_lo, idx,
 			ndet, a);
 	}
@@ -411,4 +412,25 @@ int ph2_reg_read(void *r, double *state)
 {
 	struct qreg *reg = r;
 	return ph2hip_d2h(state, qreg_hip_device_ptr(reg), reg->namp * 16);
+}
+
+int ph2_reg_gather(void *r, size_t n, const uint64_t *idx, double *out)
+{
+	struct qreg *reg = r;
+	return ph2hip_gather(qreg_hip_device_ptr(reg), reg->qb_lo, idx, n, out);
+}
+
+int ph2_reg_fill_hash(void *r, uint64_t seed)
+{
+	struct qreg *reg = r;
+	return ph2hip_fill_hash(qreg_hip_device_ptr(reg), reg->namp, seed);
+}
+
+int ph2_reg_count_unequal(void *r1, void *r2, uint64_t *count)
+{
+	struct qreg *a = r1, *b = r2;
+	if (a->namp != b->namp)
+		return -1;
+	return ph2hip_count_unequal(qreg_hip_device_ptr(a),
+		qreg_hip_device_ptr(b), a->namp, count);
 }

@@ -12,6 +12,12 @@
 typedef int (*term_cmp)(const struct circ_hamil_term *,
 	const struct circ_hamil_term *);
 
+/* One private pointer a layer above the register keeps with it (qreg_hip.c):
+ * set_aux replaces (and releases through its aux_free) the previous one;
+ * qreg_free releases the last.  Returns -1 if the register has no backend. */
+void *qreg_hip_aux(struct qreg *reg);
+int qreg_hip_set_aux(struct qreg *reg, void *aux, void (*aux_free)(void *));
+
 /* Stable sort of a term array (circ.c). */
 int circ_terms_sort(struct circ_hamil_term *t, size_t n, term_cmp cmp);
 

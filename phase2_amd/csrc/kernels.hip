@@ -992,6 +992,57 @@ __global__ __launch_bounds__(kBlock) void k_scatter_set(
 		a[idx[k]] = val[k];
 }
 
+// splitmix64 output function of x (Steele, Lea and Flood, "Fast splittable
+// pseudorandom number generators", OOPSLA 2014): a bijective 64-bit mix.
+__device__ __forceinline__ uint64_t mix64(uint64_t x)
+{
+	x = (x ^ (x >> 30)) * UINT64_C(0xbf58476d1ce4e5b9);
+	x = (x ^ (x >> 27)) * UINT64_C(0x94d049bb133111eb);
+	return x ^ (x >> 31);
+}
+
+// A double in [-1, 1) from the top 53 bits of h: k * 2^-52 - 1, exact.
+__device__ __forceinline__ double unit_of(uint64_t h)
+{
+	return add(mul((double)(h >> 11), 0x1p-52), -1.0);
+}
+
+// amp[i] = (unit_of(mix64(seed + (2i+1) g)), unit_of(mix64(seed + (2i+2) g)))
+// with g the golden-ratio increment: a reproducible dense state for checks
+// at any width without a host copy (the host restates it in numpy).
+__global__ __launch_bounds__(kBlock) void k_fill_hash(
+	double2 *__restrict__ a, uint64_t n, uint64_t seed)
+{
+	const uint64_t g = UINT64_C(0x9e3779b97f4a7c15);
+	const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+	for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+		i += stride)
+		a[i] = make_double2(unit_of(mix64(seed + (2 * i + 1) * g)),
+			unit_of(mix64(seed + (2 * i + 2) * g)));
+}
+
+// Number of components (re and im counted separately) where a and b differ
+// as IEEE doubles (-0 == +0, NaN unequal to everything): per-thread counts,
+// a wave sum, one atomic add per wave.
+__global__ __launch_bounds__(kBlock) void k_count_neq(
+	const double2 *__restrict__ a, const double2 *__restrict__ b, uint64_t n,
+	unsigned long long *__restrict__ count)
+{
+	const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+	uint32_t c = 0;
+	for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+		i += stride) {
+		const double2 u = a[i], v = b[i];
+		c += (u.x != v.x) + (u.y != v.y);
+	}
+	unsigned long long t = c;
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1)
+		t += __shfl_xor(t, off, 64);
+	if ((threadIdx.x & 63) == 0 && t)
+		atomicAdd(count, t);
+}
+
 __device__ __forceinline__ uint64_t taper_idx(uint64_t v, uint32_t n_sites)
 {
 	const uint64_t lo = (v >> 1) & ((UINT64_C(1) << (n_sites - 1)) - 1);
@@ -2190,6 +2241,37 @@ int ph2hip_norm2(const void *amp, uint32_t nqb, double *out)
 	if (ph2hip_d2h(r, partial + kRedBlocks, sizeof r))
 		return -1;
 	*out = r[0];
+	return 0;
+}
+
+int ph2hip_fill_hash(void *amp, uint64_t namp, uint64_t seed)
+{
+	if (ensure_init())
+		return -1;
+	hipLaunchKernelGGL(k_fill_hash, dim3(grid_for(namp, kBlock * 8,
+		kMaxGrid)), dim3(kBlock), 0, g_stream, (double2 *)amp, namp,
+		seed);
+	HIPCHK(hipGetLastError());
+	return 0;
+}
+
+int ph2hip_count_unequal(const void *a, const void *b, uint64_t namp,
+	uint64_t *count)
+{
+	if (ensure_init() || ensure_red())
+		return -1;
+	unsigned long long *dc = (unsigned long long *)g_red;
+	hipLaunchKernelGGL(k_fill, dim3(1), dim3(kBlock), 0, g_stream,
+		(double2 *)g_red, (uint64_t)1);
+	HIPCHK(hipGetLastError());
+	hipLaunchKernelGGL(k_count_neq, dim3(grid_for(namp, kBlock * 8,
+		kRedBlocks * 4)), dim3(kBlock), 0, g_stream,
+		(const double2 *)a, (const double2 *)b, namp, dc);
+	HIPCHK(hipGetLastError());
+	unsigned long long c = 0;
+	if (ph2hip_d2h(&c, dc, sizeof c))
+		return -1;
+	*count = c;
 	return 0;
 }
 

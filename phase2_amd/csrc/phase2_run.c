@@ -52,11 +52,16 @@ static int parse_basis(const char *s, uint32_t nqb, uint64_t *out)
 	return 0;
 }
 
-/* Blank-separated tokens "<op><qubit>", op in IXYZ.  The qubit field is
- * read like strtoul(..., 10) in the reference (phase2/phase2_run.c:60-103):
- * an optional sign, at least one digit, nothing after; a negative value
- * wraps, so only "-0" survives the qubit < nqb check.  Strings of 4096
- * characters or more are rejected, as there. */
+/* Blank-separated tokens "<op><qubit>", op in IXYZ.  Tokens are split on
+ * blanks, tabs and newlines as strtok(" \t\n") splits them, and the
+ * qubit field is read as strtoul(field, &end, 10) reads it in the
+ * reference (phase2/phase2_run.c:60-103): leading white space (only \v,
+ * \f and \r can be left inside a token) is skipped, then an optional
+ * sign and at least one digit; the value is taken modulo 2^64 and negated
+ * as an unsigned number after a '-', so "X-18446744073709551615" is qubit
+ * 1; a magnitude of 2^64 or more saturates (ERANGE) and fails the qubit <
+ * nqb check; nothing may follow the digits.  Strings of 4096 characters or
+ * more are rejected, as there. */
 static int parse_pauli(const char *s, uint32_t nqb, struct paulis *out)
 {
 	if (strlen(s) >= 4096)
@@ -77,24 +82,30 @@ static int parse_pauli(const char *s, uint32_t nqb, struct paulis *out)
 		default: return -1;
 		}
 		p++;
+		while (*p == '\v' || *p == '\f' || *p == '\r')
+			p++;
 		int neg = 0;
 		if (*p == '+' || *p == '-')
 			neg = *p++ == '-';
 		if (*p < '0' || *p > '9')
 			return -1;
 		uint64_t q = 0;
-		int big = 0;
+		int range = 0;	/* magnitude >= 2^64 */
 		while (*p >= '0' && *p <= '9') {
-			if (q > (UINT64_MAX - 9) / 10)
-				big = 1;
-			q = q * 10 + (uint64_t)(*p - '0');
+			const uint64_t d = (uint64_t)(*p - '0');
+			if (q > UINT64_MAX / 10 ||
+			    (q == UINT64_MAX / 10 && d > UINT64_MAX % 10))
+				range = 1;
+			q = q * 10 + d;
 			p++;
 		}
 		if (*p && *p != ' ' && *p != '\t' && *p != '\n')
 			return -1;
-		if (neg)
+		if (range)
+			q = UINT64_MAX;
+		else if (neg)
 			q = (uint64_t)0 - q;
-		if (big || q >= nqb)
+		if (q >= nqb)
 			return -1;
 		paulis_set(out, op, (uint32_t)q);
 	}

@@ -37,6 +37,10 @@ struct qreg_hip {
 	struct ph2hip_rot *rot;
 	size_t rot_cap;
 	struct ph2_sched sched;
+	/* state a caller above the register keeps with it (the multidet
+	 * overlap plan of circ.c), released by qreg_free */
+	void *aux;
+	void (*aux_free)(void *);
 };
 
 static int g_mode = -1;
@@ -178,10 +182,30 @@ int qreg_init(struct qreg *reg, uint32_t qb)
 	return 0;
 }
 
+void *qreg_hip_aux(struct qreg *reg)
+{
+	const struct qreg_hip *h = reg->backend;
+	return h ? h->aux : NULL;
+}
+
+int qreg_hip_set_aux(struct qreg *reg, void *aux, void (*aux_free)(void *))
+{
+	struct qreg_hip *h = reg->backend;
+	if (!h)
+		return -1;
+	if (h->aux && h->aux_free)
+		h->aux_free(h->aux);
+	h->aux = aux;
+	h->aux_free = aux_free;
+	return 0;
+}
+
 void qreg_free(struct qreg *reg)
 {
 	struct qreg_hip *h = reg->backend;
 	if (h) {
+		if (h->aux && h->aux_free)
+			h->aux_free(h->aux);
 		ph2hip_free(h->damp);
 		free(h->rot);
 		ph2_sched_free(&h->sched);
@@ -192,11 +216,25 @@ void qreg_free(struct qreg *reg)
 	reg->backend = NULL;
 }
 
+/* The reference masks an index to the register (phase2/qreg.c:35-47), so
+ * on one rank an index beyond 2^qb silently aliases a lower one; here it is
+ * an error (sticky flag, nothing read or written). */
+static int in_range(struct qreg *reg, uint64_t i, const char *what)
+{
+	if (i < reg->namp)
+		return 1;
+	struct qreg_hip *h = reg->backend;
+	h->err = -1;
+	log_error("%s: index %llu beyond %u qubits", what,
+		(unsigned long long)i, reg->qb_lo);
+	return 0;
+}
+
 void qreg_getamp(struct qreg *reg, uint64_t i, _Complex double *z)
 {
 	struct qreg_hip *h = reg->backend;
 	double v[2] = { 0.0, 0.0 };
-	if (ph2hip_get1(h->damp, i & (reg->namp - 1), v) < 0)
+	if (in_range(reg, i, "qreg_getamp") && ph2hip_get1(h->damp, i, v) < 0)
 		set_err(reg, "qreg_getamp");
 	*z = CMPLX(v[0], v[1]);
 }
@@ -204,7 +242,8 @@ void qreg_getamp(struct qreg *reg, uint64_t i, _Complex double *z)
 void qreg_setamp(struct qreg *reg, uint64_t i, _Complex double z)
 {
 	struct qreg_hip *h = reg->backend;
-	if (ph2hip_set1(h->damp, i & (reg->namp - 1), creal(z), cimag(z)) < 0)
+	if (in_range(reg, i, "qreg_setamp") &&
+	    ph2hip_set1(h->damp, i, creal(z), cimag(z)) < 0)
 		set_err(reg, "qreg_setamp");
 }
 

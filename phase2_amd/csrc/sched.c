@@ -61,8 +61,12 @@ int ph2_sched_greedy(const struct ph2hip_rot *rot, size_t n, uint32_t nqb,
 	out->npass = 0;
 	if (tile_bits > nqb)
 		tile_bits = nqb;
-	if (low_bits > tile_bits)
-		low_bits = tile_bits;
+	if (tile_bits == 0)
+		return -1;
+	/* at least one tile bit is left for the flip masks, so every pass
+	 * holds at least one term (gmax >= 1) */
+	if (low_bits >= tile_bits)
+		low_bits = tile_bits - 1;
 	const uint64_t lowmask = (UINT64_C(1) << low_bits) - 1;
 	const uint32_t gmax = tile_bits - low_bits;
 	uint64_t basis[64];

@@ -129,9 +129,10 @@ static size_t candidates(const struct ph2hip_rot *rot, size_t n,
 			if (tb < PH2HIP_TILE_MIN || (t > 0 && tb ==
 				    (TILES[t - 1] < nqb ? TILES[t - 1] : nqb)))
 				continue;
-			const uint32_t low = LM.low + dl < tb ? LM.low + dl : tb;
+			/* one tile bit at least is left for the flip masks */
+			const uint32_t low = LM.low + dl < tb ? LM.low + dl : tb - 1;
 			if (dl > 0 && low == (LM.low + dl - 1 < tb ?
-				    LM.low + dl - 1 : tb))
+				    LM.low + dl - 1 : tb - 1))
 				continue;
 			const uint64_t lowmask = (UINT64_C(1) << low) - 1;
 			const uint32_t gmax = tb - low;

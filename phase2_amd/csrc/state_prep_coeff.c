@@ -16,6 +16,7 @@
 
 #include <complex.h>
 #include <math.h>
+#include <stddef.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -33,6 +34,24 @@ struct sc_dev {
 	int valid;
 	int cb_null;
 };
+
+/* The library's own tables of a scratch.  The public struct has the
+ * reference's members only, so they live in one block in front of the
+ * alpha tuples: sc->tup_a is the block's tail, and the head is found from
+ * it. */
+struct sc_priv {
+	uint64_t *mask_a, *mask_b;	/* occupation bit masks, host */
+	struct sc_dev *dev;		/* device mirror, built on first use */
+	uint32_t tup[];			/* the alpha tuples (sc->tup_a) */
+};
+
+static struct sc_priv *priv_of(const struct state_prep_coeff_scratch *sc)
+{
+	if (!sc->tup_a)
+		return NULL;
+	return (struct sc_priv *)((char *)sc->tup_a -
+		offsetof(struct sc_priv, tup));
+}
 
 static uint64_t n_choose_k(uint32_t n, uint32_t k)
 {
@@ -88,19 +107,24 @@ int state_prep_coeff_scratch_init(struct state_prep_coeff_scratch *sc,
 	sc->Ma = (size_t)n_choose_k(n_sites, n_alpha);
 	sc->Mb = (size_t)n_choose_k(n_sites, n_beta);
 	const uint32_t ka = n_alpha ? n_alpha : 1, kb = n_beta ? n_beta : 1;
-	sc->tup_a = malloc(sizeof *sc->tup_a * sc->Ma * ka);
+	struct sc_priv *pv = calloc(1, sizeof *pv +
+		sizeof *pv->tup * sc->Ma * ka);
+	if (pv)
+		sc->tup_a = pv->tup;
 	sc->tup_b = malloc(sizeof *sc->tup_b * sc->Mb * kb);
 	sc->det_a = malloc(sizeof *sc->det_a * sc->Ma);
 	sc->det_b = malloc(sizeof *sc->det_b * sc->Mb);
-	sc->mask_a = malloc(sizeof *sc->mask_a * sc->Ma);
-	sc->mask_b = malloc(sizeof *sc->mask_b * sc->Mb);
-	if (!sc->tup_a || !sc->tup_b || !sc->det_a || !sc->det_b ||
-	    !sc->mask_a || !sc->mask_b) {
+	if (pv) {
+		pv->mask_a = malloc(sizeof *pv->mask_a * sc->Ma);
+		pv->mask_b = malloc(sizeof *pv->mask_b * sc->Mb);
+	}
+	if (!pv || !sc->tup_b || !sc->det_a || !sc->det_b ||
+	    !pv->mask_a || !pv->mask_b) {
 		log_error("scratch_init: alloc failed");
 		goto err;
 	}
-	if (fill_subsets(n_sites, n_alpha, sc->tup_a, sc->mask_a, sc->Ma) < 0 ||
-	    fill_subsets(n_sites, n_beta, sc->tup_b, sc->mask_b, sc->Mb) < 0)
+	if (fill_subsets(n_sites, n_alpha, sc->tup_a, pv->mask_a, sc->Ma) < 0 ||
+	    fill_subsets(n_sites, n_beta, sc->tup_b, pv->mask_b, sc->Mb) < 0)
 		goto err;
 	return 0;
 err:
@@ -110,19 +134,22 @@ err:
 
 void state_prep_coeff_scratch_free(struct state_prep_coeff_scratch *sc)
 {
-	struct sc_dev *d = sc->dev;
-	if (d) {
-		ph2hip_sc_destroy(d->tab);
-		free(d->ca);
-		free(d->cb);
-		free(d);
+	struct sc_priv *pv = priv_of(sc);
+	if (pv) {
+		struct sc_dev *d = pv->dev;
+		if (d) {
+			ph2hip_sc_destroy(d->tab);
+			free(d->ca);
+			free(d->cb);
+			free(d);
+		}
+		free(pv->mask_a);
+		free(pv->mask_b);
+		free(pv);
 	}
-	free(sc->tup_a);
 	free(sc->tup_b);
 	free(sc->det_a);
 	free(sc->det_b);
-	free(sc->mask_a);
-	free(sc->mask_b);
 	memset(sc, 0, sizeof *sc);
 }
 
@@ -147,7 +174,10 @@ static void subset_dets(uint32_t k, const uint32_t *tup, const double *C,
 static struct ph2hip_sc *device_tables(struct state_prep_coeff_scratch *sc,
 	const double *C_alpha, const double *C_beta)
 {
-	struct sc_dev *d = sc->dev;
+	struct sc_priv *pv = priv_of(sc);
+	if (!pv)
+		return NULL;
+	struct sc_dev *d = pv->dev;
 	const size_t na = (size_t)sc->n_sites * sc->n_alpha;
 	const size_t nb = (size_t)sc->n_sites * sc->n_beta;
 	if (!d) {
@@ -157,14 +187,14 @@ static struct ph2hip_sc *device_tables(struct state_prep_coeff_scratch *sc,
 		d->ca = malloc(sizeof(double) * (na ? na : 1));
 		d->cb = malloc(sizeof(double) * (nb ? nb : 1));
 		if (!d->ca || !d->cb ||
-		    ph2hip_sc_create(&d->tab, sc->mask_a, sc->Ma, sc->mask_b,
+		    ph2hip_sc_create(&d->tab, pv->mask_a, sc->Ma, pv->mask_b,
 			    sc->Mb) < 0) {
 			free(d->ca);
 			free(d->cb);
 			free(d);
 			return NULL;
 		}
-		sc->dev = d;
+		pv->dev = d;
 	}
 	const double *Cb = C_beta ? C_beta : C_alpha;
 	const int same = d->valid && memcmp(d->ca, C_alpha, sizeof(double) * na)

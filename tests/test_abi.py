@@ -55,3 +55,19 @@ def test_no_undefined_library_symbols():
            if not re.search(r"@(GLIBC|GCC|hip|CXXABI|GLIBCXX)", ln)
            and not ln.strip().startswith("w ")]
     assert not bad, bad
+
+
+def test_dropin_library_loads():
+    """The drop-in check library (the reference's drivers, unchanged, on
+    this library) loads without a GPU and exports its entry points and the
+    reference driver functions."""
+    import ctypes as C
+    from pathlib import Path
+    p = Path(__file__).resolve().parents[1] / "oracle" / "_ref" / \
+        "libcirc_dropin.so"
+    if not p.is_file():
+        pytest.skip("oracle/_ref/libcirc_dropin.so not built")
+    lib = C.CDLL(str(p))
+    for f in ("dropin_trott", "dropin_qdrift", "dropin_cmpsit", "trott_init",
+              "trott_simul", "trott2_simul", "qdrift_simul", "cmpsit_simul"):
+        assert hasattr(lib, f), f

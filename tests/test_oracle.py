@@ -131,6 +131,35 @@ def test_expand_inner_golden(oracle, golden):
         assert z == g[f"inner{c}"][0]
 
 
+def test_phase2_run_golden(golden):
+    """orc_phase2_run against the reference's own phase2_run outputs; the
+    strings become masks through the product's reader (pinned separately
+    against glibc strtoul in test_parser.py)."""
+    from phase2_amd import pauli_masks
+    g = golden("phase2_run.npz")
+    runs = [(["Z0"], [1.0], 0.3, "0"),
+            (["X0 Y1", "Z1 Z2"], [0.3, -0.5], 0.2, "010"),
+            ([f"X{k}" for k in range(10)], [0.1 * (k + 1) for k in range(10)],
+             0.05, "0" * 10),
+            (list(g["terms8"]), list(g["cf8"]), 0.3, "01101001")]
+    for (p, c, d, s), want in zip(runs, g["out"]):
+        xm, zm = pauli_masks(p, len(s))
+        idx = sum(1 << q for q, ch in enumerate(s) if ch == "1")
+        assert po.phase2_run(len(s), idx, xm, zm, c, d) == want
+
+
+def test_phase2_run_closed_form():
+    """exp(i phi Z0 Z1) on |00> and a product of X rotations on |0..0>."""
+    got = po.phase2_run(2, 0, [0], [3], [1.0], 0.8)
+    assert abs(got - complex(math.cos(0.8), math.sin(0.8))) < TOL
+    h = [0.1 * (k + 1) for k in range(6)]
+    xm = [1 << k for k in range(6)]
+    got = po.phase2_run(6, 0, xm, [0] * 6, h, 0.05)
+    assert abs(got - math.prod(math.cos(0.05 * v) for v in h)) < TOL
+    with pytest.raises(ValueError):
+        po.phase2_run(31, 0, [], [], [], 0.1)
+
+
 def test_golden_inputs_unchanged(golden):
     """The generator still produces the inputs the fixtures were made from."""
     import hashlib
