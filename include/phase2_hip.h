@@ -192,6 +192,20 @@ int ph2hip_gather_abs(const void *amp, uint64_t namp_total,
 int ph2hip_gather(const void *amp, uint32_t nqb, const uint64_t *idx,
 	size_t n, double *out);
 
+/* ph2hip_gather without any device allocation: the indices and gathered
+ * amplitudes pass through a fixed device array of the library (24 B per
+ * entry for PH2HIP_GATHER_NOALLOC_MAX entries, 1.5 MiB on each device the
+ * library runs on, held until the process ends) instead of buffers
+ * allocated per call; no hipMalloc or hipFree.  Same results as
+ * ph2hip_gather.  n above PH2HIP_GATHER_NOALLOC_MAX fails (-1, nothing
+ * read).  Every call on one device shares that array: calls that overlap
+ * on a device (several host threads) race.  No counterpart in the
+ * reference; for callers that cannot allocate device memory (per-step
+ * overlap loops and harnesses that forbid device allocation). */
+#define PH2HIP_GATHER_NOALLOC_MAX 65536
+int ph2hip_gather_noalloc(const void *amp, uint32_t nqb, const uint64_t *idx,
+	size_t n, double *out);
+
 /* Device-resident Slater-Condon tables: occupation masks and determinants
  * of the alpha and beta k-subsets (host arrays copied to the device). */
 struct ph2hip_sc;

@@ -33,6 +33,12 @@
 
 #include "phase2_hip.h"
 
+// Staging of ph2hip_gather_noalloc: PH2HIP_GATHER_NOALLOC_MAX indices, then
+// as many amplitudes (16 B each).  Each device holds its own copy.  In host
+// code this name is a host-side stand-in, not a device address: the address
+// comes from hipGetSymbolAddress at ph2hip_init (g_scratch below).
+static __device__ __align__(256) uint64_t scratch[3 * PH2HIP_GATHER_NOALLOC_MAX];
+
 namespace {
 
 // ---------------------------------------------------------------------------
@@ -46,6 +52,9 @@ uint64_t g_persist_wgs = 512;   // fused-pass workgroups (2 per CU by default)
 uint32_t g_ncu = 256;          // compute units of the device
 double *g_red = nullptr;       // reduction partials (device)
 size_t g_red_cap = 0;
+// device address of `scratch` per device ordinal, taken once at ph2hip_init
+constexpr int kMaxDevices = 64;
+void *g_scratch[kMaxDevices] = {};
 
 int fail(const char *fmt, ...)
 {
@@ -1757,6 +1766,15 @@ int ph2hip_init(int device)
 	if (device >= ndev)
 		return fail("device %d not present (%d devices)", device, ndev);
 	HIPCHK(hipSetDevice(device));
+	if (device < kMaxDevices && !g_scratch[device]) {
+		void *p = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&p,
+			HIP_SYMBOL(scratch));
+		if (e != hipSuccess || !p)
+			return fail("init: address of the gather scratch on "
+				"device %d: %s", device, hipGetErrorString(e));
+		g_scratch[device] = p;
+	}
 	HIPCHK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
 	g_device = device;
 	int ncu = 0;
@@ -2015,6 +2033,43 @@ int ph2hip_gather(const void *amp, uint32_t nqb, const uint64_t *idx,
 	(void)hipFree(didx);
 	(void)hipFree(dout);
 	return rc;
+}
+
+int ph2hip_gather_noalloc(const void *amp, uint32_t nqb, const uint64_t *idx,
+	size_t n, double *out)
+{
+	if (ensure_init())
+		return -1;
+	if (n == 0)
+		return 0;
+	if (n > PH2HIP_GATHER_NOALLOC_MAX)
+		return fail("gather_noalloc: %zu indices, at most %d", n,
+			PH2HIP_GATHER_NOALLOC_MAX);
+	const uint64_t namp = UINT64_C(1) << nqb;
+	for (size_t k = 0; k < n; k++)
+		if (idx[k] >= namp)
+			return fail("gather_noalloc: index %llu beyond 2^%u",
+				(unsigned long long)idx[k], nqb);
+	void *p = g_device >= 0 && g_device < kMaxDevices ?
+		g_scratch[g_device] : nullptr;
+	if (!p)
+		return fail("gather_noalloc: no scratch address for device %d",
+			g_device);
+	// indices first, amplitudes after the largest index block; the copy
+	// in overwrites what the previous call left, the kernel every output
+	uint64_t *didx = static_cast<uint64_t *>(p);
+	double2 *dout = reinterpret_cast<double2 *>(
+		didx + PH2HIP_GATHER_NOALLOC_MAX);
+	HIPCHK(hipMemcpyAsync(didx, idx, sizeof(uint64_t) * n,
+		hipMemcpyHostToDevice, g_stream));
+	hipLaunchKernelGGL(k_gather, dim3(grid_for(n, kBlock, kMaxGrid)),
+		dim3(kBlock), 0, g_stream, (const double2 *)amp, didx, dout,
+		(uint64_t)n);
+	HIPCHK(hipGetLastError());
+	HIPCHK(hipMemcpyAsync(out, dout, sizeof(double2) * n,
+		hipMemcpyDeviceToHost, g_stream));
+	HIPCHK(hipStreamSynchronize(g_stream));
+	return 0;
 }
 
 struct ph2hip_sc {
