@@ -110,6 +110,29 @@ int ph2hip_set_fused_direct(int on);
 #define PH2HIP_HALF_TILES ((1u << 12) | (1u << 13))
 int ph2hip_set_fused_half(uint32_t tile_mask);
 
+/* Persistent form of the half-tile pass at tile width 12 or 13: each
+ * workgroup walks tiles and loads the next tile's first-group cosets (amps
+ * = 8 or 16 of the 16 amplitudes of each coset) into registers while it
+ * rotates the current one; wgs_per_cu workgroups per compute unit.  amps 0
+ * returns to one workgroup per tile.  Default from PHASE2_HIP_PREFETCH12 /
+ * PHASE2_HIP_PREFETCH13 ("amps:wgs_per_cu"), else off.  Every setting
+ * gives the same state. */
+int ph2hip_set_fused_prefetch(uint32_t tile_bits, uint32_t amps,
+	uint32_t wgs_per_cu);
+
+/* Replay of repeated fused sweeps on registers of at most max_qb qubits
+ * (default 20, or PHASE2_HIP_REPLAY_QB; 0 turns it off): a
+ * ph2hip_rotate_fused call whose arguments (register, terms with their
+ * c and s, schedule) all equal an earlier call's -- a Trotter step's
+ * circ_cache block from the second step on -- has its launches captured
+ * into a hipGraph on its second occurrence and replayed from the third,
+ * so the passes of a block go out as one submission.  The state is the
+ * same whatever the setting.  Not while profiling.  Setting it drops the
+ * cached graphs.  ph2hip_replay_hits counts replays.  No counterpart in the
+ * reference. */
+int ph2hip_set_replay(uint32_t max_qb);
+uint64_t ph2hip_replay_hits(void);
+
 
 /* Shape of the fused pass over rot[0..n) without launching it (shape has 4
  * entries): shape[0] = contiguous low bits L, shape[1] = gathered
@@ -152,8 +175,8 @@ struct ph2hip_batch_job {
 /* Apply every job's rotations, bit-identical to ph2hip_rotate_plain on each
  * register alone.  Launch p applies pass p of every job that has one, so a
  * batch costs max(npass) launches instead of sum(n); with use_graph the
- * launches are captured in one hipGraph and replayed as a single submission
- * (not while profiling).  *launches (if non-NULL) gets the launch count.
+ * launches go out as one submission of the round graph below (not while
+ * profiling).  *launches (if non-NULL) gets the launch count.
  * Returns when the work is complete.  No counterpart in the reference
  * (which runs samples one after another, circ/qdrift.c:115-131). */
 int ph2hip_rotate_batched(void *amp, uint64_t namp_total,
@@ -164,7 +187,7 @@ int ph2hip_rotate_batched(void *amp, uint64_t namp_total,
  * the host while the device runs this one.  plan_build validates the jobs
  * and builds every pass descriptor on the host (no device work; the jobs'
  * arrays may change once it returns); plan_launch uploads the plan and
- * queues its launches (a hipGraph with use_graph) behind the work already
+ * queues its launches (the round graph with use_graph) behind the work already
  * on the library stream, and returns without waiting.  A plan may be freed
  * after ph2hip_synchronize (or any call that waits for the stream). */
 struct ph2hip_batch_plan;
@@ -174,6 +197,15 @@ int ph2hip_batch_plan_build(const struct ph2hip_batch_job *jobs, size_t njobs,
 int ph2hip_batch_plan_launch(void *amp, struct ph2hip_batch_plan *plan,
 	int use_graph, uint64_t *launches);
 void ph2hip_batch_plan_free(struct ph2hip_batch_plan *plan);
+
+/* Launches of batched rounds (use_graph, not while profiling) go through
+ * one graph of kernel nodes -- two per pass index, the half-tile form then
+ * the whole-tile form -- built once and replayed every round with each
+ * node's work items set anew (hipGraphExecKernelNodeSetParams; a node with
+ * no work exits at once); it is rebuilt only when a round has more pass
+ * indices than it holds or another tile width.  Counts of graph builds and
+ * replays since the process started. */
+void ph2hip_round_graph_stats(uint64_t *builds, uint64_t *replays);
 
 /* amp[idx[k]] = (val[2k], val[2k+1]) with idx < namp_total (host arrays);
  * for repeated indices the last entry wins, as sequential qreg_setamp
@@ -225,6 +257,21 @@ int ph2hip_sc_scatter(void *amp, uint32_t nqb, const struct ph2hip_sc *sc,
  * fixed tree order (phase2/state_prep_coeff.c:224-274). */
 int ph2hip_sc_inner(const void *amp, uint32_t nqb, const struct ph2hip_sc *sc,
 	uint32_t n_sites, double weight, int tapered, double *out);
+
+/* -- device-side sampler ---------------------------------------------------- */
+
+/* The draws of the qDRIFT and composite samplers, made on the device:
+ * stream k continues its xoshiro256** state state[4k .. 4k+3] (advanced in
+ * place) for ndraw[k] draws u = (next >> 11) * 2^-53 and inverts its
+ * cumulative table cdf[k][0 .. len[k]) (host arrays, the table
+ * prob_cdf_from_array_strided builds) for each; out receives the term
+ * indices, stream after stream.  The same indices and final states as
+ * xoshiro256ss_dbl01 + prob_cdf_inverse on the host (lib/xoshiro256ss.c:
+ * 72-85, include/xoshiro256ss.h:58-59, lib/prob.c:67-78), whose calls
+ * circ/qdrift.c:95-103 and circ/cmpsit.c:165-172 make per draw.  One
+ * 64-lane workgroup per stream.  No counterpart in the reference. */
+int ph2hip_sample_cdf(size_t nstream, const double *const *cdf,
+	const size_t *len, uint64_t *state, const size_t *ndraw, uint32_t *out);
 
 /* -- measurement ----------------------------------------------------------- */
 

@@ -15,8 +15,8 @@
  *   f[87]      half-tile form (0/1)
  * as y = ln(ms) - (nqb - 30) ln 2 (time per 2^30 amplitudes).  For a block of
  * terms the scheduler enumerates passes [i, j) that end where the next term
- * would widen the tile (or at the longest feasible end), for tile widths 11
- * and 12 and for low_bits, low_bits + 1 and low_bits + 2 contiguous low
+ * would widen the tile (or at the longest feasible end), for tile widths 12
+ * and 13 and for low_bits, low_bits + 1 and low_bits + 2 contiguous low
  * bits, predicts them all in one batch on the GPU and picks the cut, tile
  * and low bits of every pass with the smallest predicted total by dynamic
  * programming.  Schedules are cached
@@ -54,6 +54,14 @@ void ph2_learned_uninstall(void);
 /* Learned schedule of one block (needs ph2_learned_install). */
 int ph2_learned_schedule(const struct ph2hip_rot *rot, size_t n, uint32_t nqb,
 	struct ph2_sched *out);
+
+/* The candidate passes the scheduler chooses from for the block rot[0..n)
+ * (low_bits as for ph2_learned_install; no network needed): 4 words each,
+ * first term, end (exclusive), tile bits, contiguous low bits, grouped by
+ * ascending first term.  Returns the count, or -1 if cap is too small.
+ * For evaluations that time every candidate. */
+long ph2_learned_candidates(const struct ph2hip_rot *rot, size_t n,
+	uint32_t nqb, uint32_t low_bits, uint32_t *out, size_t cap);
 
 /* Predicted milliseconds of each pass of a schedule (ms[npass]). */
 int ph2_learned_predict(const struct ph2hip_rot *rot, size_t n, uint32_t nqb,

@@ -237,6 +237,61 @@ def sort_lex(xm, zm):
     return perm
 
 
+def xoshiro_state(seed) -> np.ndarray:
+    """The four state words after xoshiro256ss_init(seed) (host library)."""
+    st = np.zeros(4, dtype=np.uint64)
+    L = lib()
+    L.xoshiro256ss_init.restype = None
+    L.xoshiro256ss_init(_p(st, _u64p), C.c_uint64(seed))
+    return st
+
+
+class _Cdf(C.Structure):
+    _fields_ = [("y", _f64p), ("len", C.c_size_t)]
+
+
+def cdf_table(w) -> np.ndarray:
+    """The cumulative table prob_cdf_from_array_strided builds over |w|."""
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    L = lib()
+    L.prob_cdf_init.restype = C.c_int
+    L.prob_cdf_from_array_strided.restype = C.c_int
+    cdf = _Cdf()
+    if L.prob_cdf_init(C.byref(cdf), C.c_size_t(w.shape[0])) != 0:
+        raise ValueError("prob_cdf_init")
+    try:
+        if L.prob_cdf_from_array_strided(C.byref(cdf), _p(w, _f64p),
+                                         C.c_size_t(8), None) != 0:
+            raise ValueError("weights sum below DBL_EPSILON")
+        return np.ctypeslib.as_array(cdf.y, shape=(w.shape[0],)).copy()
+    finally:
+        L.prob_cdf_free(C.byref(cdf))
+
+
+def sample_cdf(tables, states, ndraws):
+    """Device draws (ph2hip_sample_cdf): for each stream k, ndraws[k]
+    indices into tables[k] (cumulative, as prob_cdf builds them) from the
+    xoshiro256** state states[k] (4 words).  Returns (list of index arrays,
+    the advanced states as an (n, 4) array)."""
+    n = len(tables)
+    tabs = [np.ascontiguousarray(t, dtype=np.float64) for t in tables]
+    ptrs = (C.POINTER(C.c_double) * max(n, 1))(*[_p(t, _f64p) for t in tabs])
+    lens = np.array([t.shape[0] for t in tabs], dtype=np.uint64)
+    nd = np.ascontiguousarray(ndraws, dtype=np.uint64)
+    st = np.ascontiguousarray(np.asarray(states, dtype=np.uint64).reshape(n, 4))
+    out = np.zeros(max(1, int(nd.sum())), dtype=np.uint32)
+    L = lib()
+    L.ph2hip_sample_cdf.restype = C.c_int
+    if L.ph2hip_sample_cdf(C.c_size_t(n), ptrs, _p(lens, _u64p),
+                           _p(st, _u64p), _p(nd, _u64p),
+                           _p(out, _u32p)) != 0:
+        L.ph2hip_strerror.restype = C.c_char_p
+        raise RuntimeError("ph2hip_sample_cdf: " +
+                           L.ph2hip_strerror().decode())
+    offs = np.concatenate([[0], np.cumsum(nd)]).astype(np.int64)
+    return [out[offs[k]:offs[k + 1]].copy() for k in range(n)], st
+
+
 class Prof(C.Structure):
     """struct ph2hip_prof: kinds 0 plain pair, 1 plain diagonal, 2 fused,
     3 batched fused."""
@@ -438,9 +493,22 @@ class BatchReg(C.Structure):
                 ("step_size", C.c_double), ("seed", C.c_uint64)]
 
 
+class BatchCmpsitReg(C.Structure):
+    """struct ph2_batch_cmpsit_reg (include/phase2_batch.h)."""
+    _fields_ = [("nqb", C.c_uint32), ("nterms", C.c_size_t),
+                ("xm", _u64p), ("zm", _u64p), ("cf", _f64p),
+                ("ndet", C.c_size_t), ("didx", _u64p), ("dcf", _f64p),
+                ("samples", C.c_size_t), ("steps", C.c_size_t),
+                ("length", C.c_size_t), ("depth", C.c_size_t),
+                ("angle_det", C.c_double), ("angle_rand", C.c_double),
+                ("seed", C.c_uint64)]
+
+
 class BatchStats(C.Structure):
+    """struct ph2_batch_stats (include/phase2_batch.h)."""
     _fields_ = [("seconds", C.c_double), ("tune_seconds", C.c_double),
-                ("prep_seconds", C.c_double), ("amp_updates", C.c_double), ("launches", C.c_uint64),
+                ("prep_seconds", C.c_double), ("draw_seconds", C.c_double),
+                ("amp_updates", C.c_double), ("launches", C.c_uint64),
                 ("passes", C.c_uint64), ("tile_bits", C.c_uint32),
                 ("low_bits", C.c_uint32)]
 
@@ -475,6 +543,49 @@ def batch_qdrift(regs, mode=BATCH_FUSED_BATCHED, tile_bits=0, low_bits=3):
                             _p(out, _f64p), C.byref(st))
     if rc != 0:
         _err("ph2_batch_qdrift")
+    res, o = [], 0
+    for r in regs:
+        res.append(out[o:o + r["samples"]].copy())
+        o += r["samples"]
+    stats = {f: getattr(st, f) for f, _ in BatchStats._fields_}
+    return res, stats
+
+
+def _batch_prep(r, keep):
+    idx = np.ascontiguousarray(r["det_idx"], dtype=np.uint64)
+    dcf = np.ascontiguousarray(
+        np.asarray(r["det_cf"], dtype=np.complex128).view(np.float64))
+    keep += [idx, dcf]
+    return idx, dcf
+
+
+def batch_cmpsit(regs, mode=BATCH_FUSED_BATCHED, tile_bits=0, low_bits=3):
+    """regs: list of dicts with hm, det_idx, det_cf, samples, steps, length,
+    depth, angle_det, angle_rand, seed (ph2_batch_cmpsit).  Returns (list of
+    per-register overlap arrays, stats)."""
+    keep = []
+    arr = (BatchCmpsitReg * len(regs))()
+    total = 0
+    for i, r in enumerate(regs):
+        hm = r["hm"]
+        xm, zm, cf = _terms(hm)
+        keep += [xm, zm, cf]
+        idx, dcf = _batch_prep(r, keep)
+        arr[i] = BatchCmpsitReg(
+            hm.nqb, len(cf), _p(xm, _u64p), _p(zm, _u64p), _p(cf, _f64p),
+            idx.shape[0], _p(idx, _u64p), _p(dcf, _f64p), r["samples"],
+            r["steps"], r["length"], r["depth"], r["angle_det"],
+            r["angle_rand"], r["seed"])
+        total += r["samples"]
+    out = np.zeros(max(total, 1), dtype=np.complex128)
+    st = BatchStats()
+    L = lib()
+    L.ph2_batch_cmpsit.restype = C.c_int
+    rc = L.ph2_batch_cmpsit(arr, C.c_size_t(len(regs)), C.c_int(mode),
+                            C.c_uint32(tile_bits), C.c_uint32(low_bits),
+                            _p(out, _f64p), C.byref(st))
+    if rc != 0:
+        _err("ph2_batch_cmpsit")
     res, o = [], 0
     for r in regs:
         res.append(out[o:o + r["samples"]].copy())

@@ -374,3 +374,86 @@ def stats():
     h, m = C.c_uint64(), C.c_uint64()
     nat.lib().ph2_learned_stats(C.byref(h), C.byref(m))
     return int(h.value), int(m.value)
+
+
+# -- evaluation against the measured optimum -----------------------------------
+
+def candidates(xm, zm, nqb, low=3):
+    """The candidate passes the learned scheduler chooses from for one block
+    (ph2_learned_candidates): an (m, 4) array of first term, end
+    (exclusive), tile bits, low bits."""
+    L = nat.lib()
+    rot = _rots(xm, zm)
+    cap = len(xm) * 2 * 3 * 16 + 16
+    out = np.zeros(4 * cap, np.uint32)
+    L.ph2_learned_candidates.restype = C.c_long
+    m = L.ph2_learned_candidates(rot, C.c_size_t(len(xm)), C.c_uint32(nqb),
+                                 C.c_uint32(low),
+                                 out.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                 C.c_size_t(cap))
+    if m < 0:
+        raise RuntimeError("ph2_learned_candidates")
+    return out[:4 * m].reshape(m, 4).copy()
+
+
+def measure_candidates(reg, xm, zm, phi, cand, reps=2):
+    """Kernel time (ms, HIP events, the least of reps) of every candidate
+    pass run on its own on a device register."""
+    L = nat.lib()
+    L.ph2_reg_rotate_sched.restype = C.c_int
+    u32p = C.POINTER(C.c_uint32)
+    best = np.full(len(cand), np.inf)
+    chunk = 20000
+    for _ in range(reps):
+        for c0 in range(0, len(cand), chunk):
+            part = cand[c0:c0 + chunk]
+            nat.prof_enable(True)
+            nat.prof_records(reset=True)
+            for i, j, tb, low in part:
+                bx = np.ascontiguousarray(xm[i:j])
+                bz = np.ascontiguousarray(zm[i:j])
+                bp = np.ascontiguousarray(phi[i:j])
+                sch = np.array([[j - i], [tb], [low]], np.uint32)
+                rc = L.ph2_reg_rotate_sched(
+                    C.c_void_p(reg.h), C.c_size_t(j - i),
+                    bx.ctypes.data_as(C.POINTER(C.c_uint64)),
+                    bz.ctypes.data_as(C.POINTER(C.c_uint64)),
+                    bp.ctypes.data_as(C.POINTER(C.c_double)),
+                    sch[0].ctypes.data_as(u32p), sch[1].ctypes.data_as(u32p),
+                    sch[2].ctypes.data_as(u32p), C.c_size_t(1))
+                if rc != 0:
+                    raise RuntimeError("ph2_reg_rotate_sched: "
+                                       f"{L.ph2_strerror()}")
+            rec = nat.prof_records(reset=True)
+            nat.prof_enable(False)
+            rec = rec[rec["kind"] == 2]
+            if len(rec) != len(part):
+                raise RuntimeError(f"{len(rec)} records for {len(part)} "
+                                   "candidates")
+            best[c0:c0 + len(part)] = np.minimum(best[c0:c0 + len(part)],
+                                                 rec["ms"])
+    return best
+
+
+def dp_schedule(cand, cost, n):
+    """Cheapest cut of terms [0, n) into candidate passes (shortest path
+    over cut points, candidates grouped by ascending first term); returns
+    (schedule arrays, total cost)."""
+    best = np.full(n + 1, np.inf)
+    best[0] = 0.0
+    frm = np.full(n + 1, -1, np.int64)
+    for c, (i, j, tb, low) in enumerate(cand):
+        v = best[i] + cost[c]
+        if v < best[j]:
+            best[j] = v
+            frm[j] = c
+    lens, tbs, lbs = [], [], []
+    j = n
+    while j > 0:
+        i, _, tb, low = cand[frm[j]]
+        lens.append(j - i)
+        tbs.append(tb)
+        lbs.append(low)
+        j = i
+    return (np.array(lens[::-1], np.uint32), np.array(tbs[::-1], np.uint32),
+            np.array(lbs[::-1], np.uint32)), float(best[n])

@@ -27,7 +27,7 @@ static int cmp_abs_desc(const struct circ_hamil_term *a,
 	return x > y ? -1 : (x < y ? 1 : 0);
 }
 
-static void ranct_free(struct cmpsit_ranct *r)
+void cmpsit_ranct_free(struct cmpsit_ranct *r)
 {
 	circ_hamil_free(&r->hm_det);
 	circ_hamil_free(&r->hm_ran);
@@ -35,7 +35,7 @@ static void ranct_free(struct cmpsit_ranct *r)
 	prob_cdf_free(&r->cdf);
 }
 
-static int ranct_init(struct cmpsit_ranct *r, const struct circ_hamil *hm,
+int cmpsit_ranct_init(struct cmpsit_ranct *r, const struct circ_hamil *hm,
 	const struct cmpsit_data *dt)
 {
 	memset(r, 0, sizeof *r);
@@ -68,7 +68,7 @@ static int ranct_init(struct cmpsit_ranct *r, const struct circ_hamil *hm,
 	return 0;
 err:
 	circ_hamil_free(&tmp);
-	ranct_free(r);
+	cmpsit_ranct_free(r);
 	return -1;
 }
 
@@ -80,7 +80,7 @@ int cmpsit_init(struct cmpsit *cp, const struct cmpsit_data *dt,
 		return -1;
 	cp->dt = *dt;
 	cp->sw = sw;
-	if (ranct_init(&cp->ranct, &cp->ct.hm, dt) < 0) {
+	if (cmpsit_ranct_init(&cp->ranct, &cp->ct.hm, dt) < 0) {
 		circ_free(&cp->ct);
 		return -1;
 	}
@@ -94,7 +94,7 @@ int cmpsit_init(struct cmpsit *cp, const struct cmpsit_data *dt,
 
 void cmpsit_free(struct cmpsit *cp)
 {
-	ranct_free(&cp->ranct);
+	cmpsit_ranct_free(&cp->ranct);
 	circ_free(&cp->ct);
 }
 

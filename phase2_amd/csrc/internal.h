@@ -21,6 +21,15 @@ int qreg_hip_set_aux(struct qreg *reg, void *aux, void (*aux_free)(void *));
 /* Stable sort of a term array (circ.c). */
 int circ_terms_sort(struct circ_hamil_term *t, size_t n, term_cmp cmp);
 
+/* The composite sampler's split of H (cmpsit.c): the `length` largest |c|
+ * terms in lex order (hm_det), the rest by |c| descending (hm_ran) with its
+ * CDF, and room for one draw (hm_smpl).  Shared with the batched driver. */
+struct cmpsit_ranct;
+struct cmpsit_data;
+int cmpsit_ranct_init(struct cmpsit_ranct *r, const struct circ_hamil *hm,
+	const struct cmpsit_data *dt);
+void cmpsit_ranct_free(struct cmpsit_ranct *r);
+
 /* Sign with a DBL_EPSILON dead zone: -1, +1, or 0 for |a| < DBL_EPSILON
  * (behaviour of circ/internal.h:26-32 of the reference). */
 static inline double signof(double a)

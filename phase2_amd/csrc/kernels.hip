@@ -338,76 +338,80 @@ __device__ __forceinline__ double flip_sign(double v, uint32_t f)
 }
 
 // One pair update in place: (x, y) of one coset pair, c uniform, si the
-// per-thread signed sine.  ODD:  x <- c x + si y,  y <- c y - si x;
-// even:  x <- c x + si (i y),  y <- c y + si (i x), per component as in
-// rot_pair.  Each result is round(round(c u) +- round(si v)), the same
-// roundings as add(mul(c, u), mul(+-si, v)) (negation is exact).  Written
-// as one block with the four values read and written in place: as plain
-// code the compiler gives each term's results new registers and then
-// copies the whole coset back once per term at the loop back-edge.  The
-// cosine products come first, so si (computed just before) is not waited
-// on by the first four multiplies.
+// per-thread signed sine, si = sb * sg with sg = +-1.0 uniform.  ODD:
+// x <- c x + si y,  y <- c y - si x;  even:  x <- c x + si (i y),
+// y <- c y + si (i x), per component as in rot_pair.  Each result is
+// round(round(c u) +- round(si v)), the same roundings as
+// add(mul(c, u), mul(+-si, v)) (negation is exact).  Written as one block
+// with the four values read and written in place: as plain code the
+// compiler gives each term's results new registers and then copies the
+// whole coset back once per term at the loop back-edge.  The real and the
+// imaginary halves are two independent 2x2 updates sharing four
+// temporaries, and si is formed inside the block, so a pair holds five
+// temporaries at most (the scheduler would otherwise form the signed
+// sines of all pairs of a term early and keep them live).
 template <bool ODD>
 __device__ __forceinline__ void pair_inplace(double &xr, double &xi,
-	double &yr, double &yi, double c, double si)
+	double &yr, double &yi, double c, double sb, double sg)
 {
-	double t0, t1, t2, t3, t4, t5, t6, t7;
+	double si, t0, t1, t2, t3;
 	if (ODD)
 		__asm__(
+			"v_mul_f64 %[si], %[sg], %[sb]\n\t"
 			"v_mul_f64 %[t0], %[c], %[xr]\n\t"
 			"v_mul_f64 %[t2], %[c], %[yr]\n\t"
-			"v_mul_f64 %[t4], %[c], %[xi]\n\t"
-			"v_mul_f64 %[t6], %[c], %[yi]\n\t"
 			"v_mul_f64 %[t1], %[si], %[yr]\n\t"
 			"v_mul_f64 %[t3], %[si], %[xr]\n\t"
-			"v_mul_f64 %[t5], %[si], %[yi]\n\t"
-			"v_mul_f64 %[t7], %[si], %[xi]\n\t"
 			"v_add_f64 %[xr], %[t0], %[t1]\n\t"
 			"v_add_f64 %[yr], %[t2], -%[t3]\n\t"
-			"v_add_f64 %[xi], %[t4], %[t5]\n\t"
-			"v_add_f64 %[yi], %[t6], -%[t7]"
+			"v_mul_f64 %[t0], %[c], %[xi]\n\t"
+			"v_mul_f64 %[t2], %[c], %[yi]\n\t"
+			"v_mul_f64 %[t1], %[si], %[yi]\n\t"
+			"v_mul_f64 %[t3], %[si], %[xi]\n\t"
+			"v_add_f64 %[xi], %[t0], %[t1]\n\t"
+			"v_add_f64 %[yi], %[t2], -%[t3]"
 			: [xr] "+v"(xr), [xi] "+v"(xi), [yr] "+v"(yr),
-			  [yi] "+v"(yi), [t0] "=&v"(t0), [t1] "=&v"(t1),
-			  [t2] "=&v"(t2), [t3] "=&v"(t3), [t4] "=&v"(t4),
-			  [t5] "=&v"(t5), [t6] "=&v"(t6), [t7] "=&v"(t7)
-			: [c] "s"(c), [si] "v"(si));
+			  [yi] "+v"(yi), [si] "=&v"(si), [t0] "=&v"(t0),
+			  [t1] "=&v"(t1), [t2] "=&v"(t2), [t3] "=&v"(t3)
+			: [c] "s"(c), [sb] "v"(sb), [sg] "s"(sg));
 	else
 		__asm__(
+			"v_mul_f64 %[si], %[sg], %[sb]\n\t"
 			"v_mul_f64 %[t0], %[c], %[xr]\n\t"
-			"v_mul_f64 %[t2], %[c], %[xi]\n\t"
-			"v_mul_f64 %[t4], %[c], %[yr]\n\t"
-			"v_mul_f64 %[t6], %[c], %[yi]\n\t"
+			"v_mul_f64 %[t2], %[c], %[yi]\n\t"
 			"v_mul_f64 %[t1], %[si], %[yi]\n\t"
-			"v_mul_f64 %[t3], %[si], %[yr]\n\t"
-			"v_mul_f64 %[t5], %[si], %[xi]\n\t"
-			"v_mul_f64 %[t7], %[si], %[xr]\n\t"
+			"v_mul_f64 %[t3], %[si], %[xr]\n\t"
 			"v_add_f64 %[xr], %[t0], -%[t1]\n\t"
-			"v_add_f64 %[xi], %[t2], %[t3]\n\t"
-			"v_add_f64 %[yr], %[t4], -%[t5]\n\t"
-			"v_add_f64 %[yi], %[t6], %[t7]"
+			"v_add_f64 %[yi], %[t2], %[t3]\n\t"
+			"v_mul_f64 %[t0], %[c], %[xi]\n\t"
+			"v_mul_f64 %[t2], %[c], %[yr]\n\t"
+			"v_mul_f64 %[t1], %[si], %[yr]\n\t"
+			"v_mul_f64 %[t3], %[si], %[xi]\n\t"
+			"v_add_f64 %[xi], %[t0], %[t1]\n\t"
+			"v_add_f64 %[yr], %[t2], -%[t3]"
 			: [xr] "+v"(xr), [xi] "+v"(xi), [yr] "+v"(yr),
-			  [yi] "+v"(yi), [t0] "=&v"(t0), [t1] "=&v"(t1),
-			  [t2] "=&v"(t2), [t3] "=&v"(t3), [t4] "=&v"(t4),
-			  [t5] "=&v"(t5), [t6] "=&v"(t6), [t7] "=&v"(t7)
-			: [c] "s"(c), [si] "v"(si));
+			  [yi] "+v"(yi), [si] "=&v"(si), [t0] "=&v"(t0),
+			  [t1] "=&v"(t1), [t2] "=&v"(t2), [t3] "=&v"(t3)
+			: [c] "s"(c), [sb] "v"(sb), [sg] "s"(sg));
 }
 
-// Diagonal update in place: a <- c a + sj (i a) = (c re - sj im,
-// c im + sj re), the roundings of rot_diag.
+// Diagonal update in place: a <- c a + sj (i a) with sj = sb * sg, i.e.
+// (c re - sj im, c im + sj re), the roundings of rot_diag.
 __device__ __forceinline__ void diag_inplace(double &re, double &im,
-	double c, double sj)
+	double c, double sb, double sg)
 {
-	double t0, t1, t2, t3;
+	double sj, t0, t1, t2, t3;
 	__asm__(
+		"v_mul_f64 %[sj], %[sg], %[sb]\n\t"
 		"v_mul_f64 %[t0], %[c], %[re]\n\t"
 		"v_mul_f64 %[t2], %[c], %[im]\n\t"
 		"v_mul_f64 %[t1], %[sj], %[im]\n\t"
 		"v_mul_f64 %[t3], %[sj], %[re]\n\t"
 		"v_add_f64 %[re], %[t0], -%[t1]\n\t"
 		"v_add_f64 %[im], %[t2], %[t3]"
-		: [re] "+v"(re), [im] "+v"(im), [t0] "=&v"(t0),
+		: [re] "+v"(re), [im] "+v"(im), [sj] "=&v"(sj), [t0] "=&v"(t0),
 		  [t1] "=&v"(t1), [t2] "=&v"(t2), [t3] "=&v"(t3)
-		: [c] "s"(c), [sj] "v"(sj));
+		: [c] "s"(c), [sb] "v"(sb), [sg] "s"(sg));
 }
 
 // +-1.0 with the sign bit f (an exact multiplier)
@@ -416,24 +420,32 @@ __device__ __forceinline__ double unit_sign(uint32_t f)
 	return __hiloint2double(0x3ff00000 | (int)(f << 31), 0);
 }
 
-// One term with compile-time flip pattern XT on a coset of 2^D registers.
-// r4 = y + 2*par (mod 4) with par = bpt ^ parity(w & ztp).  Odd y: the
-// partner enters as itself and s_j = -s_i; even y: as i*partner and
-// s_j = s_i.  s_i = s with its sign flipped by par ^ bit1(y); sb already
-// carries bpt ^ bit1(y), so per pair only parity(w & ztp) (uniform) flips.
+// One term with compile-time flip pattern XT on a coset of 2^D amplitudes
+// held as (re, im) register quads, so that the 16-byte HBM and LDS moves
+// read and write them in place (with re and im in separate arrays the
+// compiler pairs them up with two copies per move).  r4 = y + 2*par (mod 4)
+// with par = bpt ^ parity(w & ztp).  Odd y: the partner enters as itself
+// and s_j = -s_i; even y: as i*partner and s_j = s_i.  s_i = s with its
+// sign flipped by par ^ bit1(y); sb already carries bpt ^ bit1(y), so per
+// pair only parity(w & ztp) (uniform) flips.
 template <int D, int XT, bool ODD>
-__device__ __forceinline__ void coset_term(double (&re)[1 << D],
-	double (&im)[1 << D], uint32_t ztp, double sb, double c)
+__device__ __forceinline__ void coset_term(v2d (&a)[1 << D], uint32_t ztp,
+	double sb, double c)
 {
 	constexpr int H = 31 - __builtin_clz(XT);
 #pragma unroll
 	for (int q = 0; q < (1 << (D - 1)); q++) {
 		const int w = ((q >> H) << (H + 1)) | (q & ((1 << H) - 1));
+		double xr = a[w].x, xi = a[w].y, yr = a[w ^ XT].x,
+			yi = a[w ^ XT].y;
 		// per pair only the uniform sign (scalar unit) changes: one
 		// exact multiply by +-1.0 held in scalar registers
-		const double si = mul(sb,
+		pair_inplace<ODD>(xr, xi, yr, yi, c, sb,
 			unit_sign(__popc(w & ztp) & 1u));
-		pair_inplace<ODD>(re[w], im[w], re[w ^ XT], im[w ^ XT], c, si);
+		a[w].x = xr;
+		a[w].y = xi;
+		a[w ^ XT].x = yr;
+		a[w ^ XT].y = yi;
 	}
 }
 
@@ -442,17 +454,17 @@ __device__ __forceinline__ void coset_term(double (&re)[1 << D],
 // unchanged: such a path makes the compiler copy all 2^(D+1) coset
 // registers once per term at the loop back-edge.
 template <int D, bool ODD, int XT = 1>
-__device__ __forceinline__ void coset_dispatch(double (&re)[1 << D],
-	double (&im)[1 << D], uint32_t xtp, uint32_t ztp, double sb, double c)
+__device__ __forceinline__ void coset_dispatch(v2d (&a)[1 << D],
+	uint32_t xtp, uint32_t ztp, double sb, double c)
 {
 	if constexpr (XT == (1 << D) - 1) {
-		coset_term<D, XT, ODD>(re, im, ztp, sb, c);
+		coset_term<D, XT, ODD>(a, ztp, sb, c);
 	} else {
 		if (xtp == XT) {
-			coset_term<D, XT, ODD>(re, im, ztp, sb, c);
+			coset_term<D, XT, ODD>(a, ztp, sb, c);
 			return;
 		}
-		coset_dispatch<D, ODD, XT + 1>(re, im, xtp, ztp, sb, c);
+		coset_dispatch<D, ODD, XT + 1>(a, xtp, ztp, sb, c);
 	}
 }
 
@@ -468,9 +480,8 @@ __device__ __forceinline__ uint64_t tile_base(const FusedPass &P, uint64_t t)
 // The terms of one micro-group (P.t[first .. first + count), fcw) on one
 // coset of 2^D registers with base coordinate bu in the tile at base.
 template <int D>
-__device__ __forceinline__ void group_terms(double (&re)[1 << D],
-	double (&im)[1 << D], const FusedPass &P, uint32_t fcw, uint32_t bu,
-	uint64_t base)
+__device__ __forceinline__ void group_terms(v2d (&a)[1 << D],
+	const FusedPass &P, uint32_t fcw, uint32_t bu, uint64_t base)
 {
 	const uint32_t kfirst = fcw & 0xffu;
 	const uint32_t kend = kfirst + ((fcw >> 8) & 0xffu);
@@ -493,24 +504,33 @@ __device__ __forceinline__ void group_terms(double (&re)[1 << D],
 			// diagonal: a <- c a + s_j (i a), s_j = +-s
 			const double sb = flip_sign(ts, bpt);
 #pragma unroll
-			for (int w = 0; w < (1 << D); w++)
-				diag_inplace(re[w], im[w], c, mul(sb,
-					unit_sign(__popc(w & ztp) & 1u)));
+			for (int w = 0; w < (1 << D); w++) {
+				double re = a[w].x, im = a[w].y;
+				diag_inplace(re, im, c, sb,
+					unit_sign(__popc(w & ztp) & 1u));
+				a[w].x = re;
+				a[w].y = im;
+			}
 		} else {
 			const double sb = flip_sign(ts,
 				bpt ^ ((ty >> 1) & 1u));
 			if (ty & 1)
-				coset_dispatch<D, true>(re, im, xtp,
-					ztp, sb, c);
+				coset_dispatch<D, true>(a, xtp, ztp, sb,
+					c);
 			else
-				coset_dispatch<D, false>(re, im, xtp,
-					ztp, sb, c);
+				coset_dispatch<D, false>(a, xtp, ztp, sb,
+					c);
 		}
 		tz = tz_n;
 		c = c_n;
 		ts = ts_n;
 		tzt = tzt_n;
 		meta = meta_n;
+		// each quad stays one register quad across terms: without this
+		// the D = 3 coset was copied whole at the loop back-edge
+#pragma unroll
+		for (int w = 0; w < (1 << D); w++)
+			__asm__ volatile("" : "+v"(a[w]));
 	}
 }
 
@@ -581,14 +601,11 @@ __device__ __forceinline__ void tile_groups(v2d *__restrict__ a, v2d *tile,
 			                             // the per-access XOR
 			const uint64_t abu = (gload || gstore) ?
 				hbm_index(P, base, bu, L, g) : 0;
-			double re[1 << D], im[1 << D];
+			v2d cs[1 << D];
 			if (gload) {
 #pragma unroll
-				for (int w = 0; w < (1 << D); w++) {
-					const v2d v = a[abu ^ P.dcw[0][w]];
-					re[w] = v.x;
-					im[w] = v.y;
-				}
+				for (int w = 0; w < (1 << D); w++)
+					cs[w] = a[abu ^ P.dcw[0][w]];
 			} else {
 #pragma unroll
 				for (int w = 0; w < (1 << D); w++) {
@@ -597,22 +614,16 @@ __device__ __forceinline__ void tile_groups(v2d *__restrict__ a, v2d *tile,
 					for (int i = 0; i < D; i++)
 						if (w >> i & 1)
 							cw16 ^= gs[i];
-					const v2d v = *reinterpret_cast<const v2d *>(
+					cs[w] = *reinterpret_cast<const v2d *>(
 						reinterpret_cast<const char *>(tile) +
 						(sbu16 ^ cw16));
-					re[w] = v.x;
-					im[w] = v.y;
 				}
 			}
-			group_terms<D>(re, im, P, fcw, bu, base);
+			group_terms<D>(cs, P, fcw, bu, base);
 			if (gstore) {
 #pragma unroll
-				for (int w = 0; w < (1 << D); w++) {
-					v2d v;
-					v.x = re[w];
-					v.y = im[w];
-					a[abu ^ P.dcw[1][w]] = v;
-				}
+				for (int w = 0; w < (1 << D); w++)
+					a[abu ^ P.dcw[1][w]] = cs[w];
 			} else {
 #pragma unroll
 				for (int w = 0; w < (1 << D); w++) {
@@ -621,12 +632,9 @@ __device__ __forceinline__ void tile_groups(v2d *__restrict__ a, v2d *tile,
 					for (int i = 0; i < D; i++)
 						if (w >> i & 1)
 							cw16 ^= gs[i];
-					v2d v;
-					v.x = re[w];
-					v.y = im[w];
 					*reinterpret_cast<v2d *>(
 						reinterpret_cast<char *>(tile) +
-						(sbu16 ^ cw16)) = v;
+						(sbu16 ^ cw16)) = cs[w];
 				}
 			}
 		}
@@ -658,42 +666,37 @@ __device__ __forceinline__ void half_offsets(const FusedGroup &G, uint32_t h,
 	gs[3] = half_slot<TB>(uw1 >> 16, h) << 4;
 }
 
-// All micro-groups of a pass on one tile with half a tile of LDS: the
-// first group reads its cosets from HBM, the last writes them to HBM, and
-// between groups q and q+1 the cosets move through the buffer in two
-// rounds, one per half of the tile (split bit xh of group q; the threads
-// of one half are the waves whose coset-index bit xk is that half).  One
-// coset per thread: 2^(TB - D) threads.
+// All micro-groups of a pass on one tile with half a tile of LDS, from the
+// first group's cosets in cs (coset base bu of group 0, read from HBM by
+// the caller) to the last group's, which go back to HBM.  Between groups q
+// and q+1 the cosets move through the buffer in two rounds, one per half of
+// the tile (split bit xh of group q; the threads of one half are the waves
+// whose coset-index bit xk is that half).  One coset per thread: 2^(TB - D)
+// threads.  fresh: the buffer has no earlier readers in this workgroup (its
+// first tile), so the first exchange needs no barrier before writing.
 template <int TB, int D>
-__device__ __forceinline__ void tile_groups_half(v2d *__restrict__ a,
-	v2d *buf, const FusedPass &P, uint64_t base, uint32_t tid)
+__device__ __forceinline__ void half_groups(v2d *__restrict__ a, v2d *buf,
+	const FusedPass &P, uint64_t base, uint32_t tid, v2d (&cs)[1 << D],
+	uint32_t bu, bool fresh)
 {
 	// the split index bits (the top two) must select whole 64-lane waves
 	static_assert(TB - D >= 8 && TB - D <= 10, "2^(TB-D) threads, 256..1024");
 	const uint32_t ngroups = pass_ngroups(P);
 	const uint32_t L = pass_L(P), g = pass_g(P);
-	uint32_t bu = coset_base<TB, D>(tid, P.grp[0].jposw[0],
-		P.grp[0].jposw[1], P.grp[0].jposw[2]);
-	double re[1 << D], im[1 << D];
-	{
-		const uint64_t abu = hbm_index(P, base, bu, L, g);
-#pragma unroll
-		for (int w = 0; w < (1 << D); w++) {
-			const v2d v = a[abu ^ P.dcw[0][w]];
-			re[w] = v.x;
-			im[w] = v.y;
-		}
-	}
 	for (uint32_t gi = 0;; gi++) {
 		const uint32_t fcw = P.grp[gi].fcw;
-		group_terms<D>(re, im, P, fcw, bu, base);
+		group_terms<D>(cs, P, fcw, bu, base);
 		if (gi + 1 == ngroups)
 			break;
 		const FusedGroup &N = P.grp[gi + 1];
 		const uint32_t h = (fcw >> 16) & 0xffu, xk = fcw >> 24;
 		const uint32_t half = __builtin_amdgcn_readfirstlane(
 			(tid >> xk) & 1u);
-		const uint32_t bn = coset_base<TB, D>(tid, N.jposw[0],
+		// the thread index is re-read per exchange: hoisted out of
+		// the loop, its bit extractions would hold 8 VGPRs throughout
+		uint32_t jt = tid;
+		__asm__ volatile("" : "+v"(jt));
+		const uint32_t bn = coset_base<TB, D>(jt, N.jposw[0],
 			N.jposw[1], N.jposw[2]);
 		uint32_t gw[4], gr[4];
 		half_offsets<TB>(P.grp[gi], h, gw);
@@ -703,8 +706,8 @@ __device__ __forceinline__ void tile_groups_half(v2d *__restrict__ a,
 		__asm__("" : "+v"(sw), "+v"(sr));
 		for (uint32_t q = 0; q < 2; q++) {
 			// the buffer's previous readers are done (it has none before
-			// the first exchange: one tile per workgroup)
-			if (gi > 0 || q > 0)
+			// the first exchange of a workgroup's first tile)
+			if (gi > 0 || q > 0 || !fresh)
 				__syncthreads();
 			if (half == q) {
 #pragma unroll
@@ -714,12 +717,9 @@ __device__ __forceinline__ void tile_groups_half(v2d *__restrict__ a,
 					for (int i = 0; i < D; i++)
 						if (w >> i & 1)
 							c16 ^= gw[i];
-					v2d v;
-					v.x = re[w];
-					v.y = im[w];
 					*reinterpret_cast<v2d *>(
 						reinterpret_cast<char *>(buf) +
-						(sw ^ c16)) = v;
+						(sw ^ c16)) = cs[w];
 				}
 			}
 			__syncthreads();
@@ -731,11 +731,9 @@ __device__ __forceinline__ void tile_groups_half(v2d *__restrict__ a,
 					for (int i = 0; i < D; i++)
 						if (w >> i & 1)
 							c16 ^= gr[i];
-					const v2d v = *reinterpret_cast<const v2d *>(
+					cs[w] = *reinterpret_cast<const v2d *>(
 						reinterpret_cast<const char *>(buf) +
 						(sr ^ c16));
-					re[w] = v.x;
-					im[w] = v.y;
 				}
 			}
 		}
@@ -743,12 +741,24 @@ __device__ __forceinline__ void tile_groups_half(v2d *__restrict__ a,
 	}
 	const uint64_t abu = hbm_index(P, base, bu, L, g);
 #pragma unroll
-	for (int w = 0; w < (1 << D); w++) {
-		v2d v;
-		v.x = re[w];
-		v.y = im[w];
-		a[abu ^ P.dcw[1][w]] = v;
-	}
+	for (int w = 0; w < (1 << D); w++)
+		a[abu ^ P.dcw[1][w]] = cs[w];
+}
+
+// One tile in the half-tile form: the first group's cosets from HBM, then
+// half_groups.
+template <int TB, int D>
+__device__ __forceinline__ void tile_groups_half(v2d *__restrict__ a,
+	v2d *buf, const FusedPass &P, uint64_t base, uint32_t tid)
+{
+	const uint32_t bu = coset_base<TB, D>(tid, P.grp[0].jposw[0],
+		P.grp[0].jposw[1], P.grp[0].jposw[2]);
+	const uint64_t abu = hbm_index(P, base, bu, pass_L(P), pass_g(P));
+	v2d cs[1 << D];
+#pragma unroll
+	for (int w = 0; w < (1 << D); w++)
+		cs[w] = a[abu ^ P.dcw[0][w]];
+	half_groups<TB, D>(a, buf, P, base, tid, cs, bu, true);
 }
 
 // Direct HBM->LDS tile loads (gfx950 global_load_lds_dwordx4): the data
@@ -836,6 +846,17 @@ __device__ __forceinline__ void fused_tile(v2d *__restrict__ a,
 		store_tile<TB>(a, tile, P, bo, tid);
 }
 
+// Waves per SIMD that the LDS of the whole-tile form allows (160 KiB per
+// CU, 2^TB x 16 B per 256-thread workgroup), given to the compiler as its
+// register target: left free it spends registers that cost occupancy (114
+// VGPRs at tile 11, where five workgroups fit the LDS).
+constexpr int lds_waves(int TB)
+{
+	// 160 KiB / (2^TB * 16 B) workgroups, one wave per SIMD each; tiles
+	// 12 and 13 fit at most two workgroups, so no target there
+	return TB > 11 ? 1 : (10240 >> TB) < 8 ? (10240 >> TB) : 8;
+}
+
 // Workgroups are dealt to the 8 XCDs round-robin (workgroup w to XCD
 // w mod 8).  Remap so that each XCD takes a contiguous range of tiles:
 // workgroup w = 8 l + x runs tile start(x) + l, where XCD x owns
@@ -850,7 +871,9 @@ __device__ __forceinline__ uint64_t xcd_tile(uint64_t w, uint64_t ntiles)
 // One tile per workgroup (grid = number of tiles); HBM latency is hidden
 // by the co-resident workgroups of the CU.
 template <int TB, int D>
-__global__ __launch_bounds__(kBlock) void k_rot_fused(
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(lds_waves(TB))))
+void k_rot_fused(
 	double2 *__restrict__ amp, const FusedPass P, uint64_t ntiles)
 {
 	__shared__ v2d tile[1u << TB];
@@ -879,6 +902,60 @@ __global__ __launch_bounds__((1 << TB) >> D) void k_rot_fused_half(
 		tile_base(P, t), threadIdx.x);
 }
 
+// Persistent half-tile form with the next tile's first-group cosets loaded
+// into registers while the current tile is rotated: PF of the 2^D
+// amplitudes of each coset (the rest are loaded when the tile starts), so
+// each workgroup keeps HBM reads in flight through its compute phase
+// instead of only at its start.  Workgroup b walks virtual workgroups
+// b, b + G, b + 2G, ... (G = gridDim.x, a multiple of 8, so each stays on
+// its XCD and the XCD's workgroups take neighbouring tiles, as xcd_tile
+// orders them).  Every workgroup leaves the loop once its next virtual
+// index reaches ntiles.
+template <int TB, int D, int PF>
+__global__ __launch_bounds__((1 << TB) >> D)
+void k_rot_fused_hpipe(double2 *__restrict__ amp, const FusedPass P,
+	uint64_t ntiles)
+{
+	static_assert(PF > 0 && PF <= (1 << D), "prefetched amplitudes");
+	__shared__ v2d buf[1u << (TB - 1)];
+	v2d *__restrict__ a = reinterpret_cast<v2d *>(amp);
+	const uint32_t tid = threadIdx.x;
+	uint64_t v = blockIdx.x;
+	if (v >= ntiles)
+		return;           // whole workgroup: no barrier is left waiting
+	const uint32_t bu0 = coset_base<TB, D>(tid, P.grp[0].jposw[0],
+		P.grp[0].jposw[1], P.grp[0].jposw[2]);
+	// HBM offset of the first group's coset base within any tile
+	// (hbm_index is base ^ a function of bu)
+	const uint64_t off0 = hbm_index(P, 0, bu0, pass_L(P), pass_g(P));
+	uint64_t base = tile_base(P, xcd_tile(v, ntiles));
+	v2d cs[1 << D], pre[PF];
+#pragma unroll
+	for (int w = 0; w < PF; w++)
+		pre[w] = a[(base ^ off0) ^ P.dcw[0][w]];
+	for (bool fresh = true;; fresh = false) {
+#pragma unroll
+		for (int w = 0; w < PF; w++)
+			cs[w] = pre[w];
+#pragma unroll
+		for (int w = PF; w < (1 << D); w++)
+			cs[w] = a[(base ^ off0) ^ P.dcw[0][w]];
+		const uint64_t vn = v + gridDim.x;
+		const bool more = vn < ntiles;
+		const uint64_t bn = more ? tile_base(P, xcd_tile(vn, ntiles)) : 0;
+		if (more) {
+#pragma unroll
+			for (int w = 0; w < PF; w++)
+				pre[w] = a[(bn ^ off0) ^ P.dcw[0][w]];
+		}
+		half_groups<TB, D>(a, buf, P, base, tid, cs, bu0, fresh);
+		if (!more)
+			break;
+		v = vn;
+		base = bn;
+	}
+}
+
 // Batched pass: pass p of many registers in one launch.  Work item i covers
 // tiles [tile0_i, tile0_{i+1}) of register i (amplitudes at amp_off_i) with
 // descriptor desc_i; a workgroup finds its item by a uniform binary search.
@@ -892,6 +969,7 @@ struct BatchItem {
 // LDS and one coset per thread, as k_rot_fused_half).
 template <int TB, int D, bool HALF = false>
 __global__ __launch_bounds__(HALF ? (1 << TB) >> D : kBlock)
+__attribute__((amdgpu_waves_per_eu(HALF ? 1 : lds_waves(TB))))
 void k_rot_fused_batch(
 	double2 *__restrict__ amp, const FusedPass *__restrict__ descs,
 	const BatchItem *__restrict__ items, uint32_t nitems, uint64_t ntiles)
@@ -1050,6 +1128,82 @@ __global__ __launch_bounds__(kBlock) void k_count_neq(
 		t += __shfl_xor(t, off, 64);
 	if ((threadIdx.x & 63) == 0 && t)
 		atomicAdd(count, t);
+}
+
+// ---------------------------------------------------------------------------
+// device-side sampler (qDRIFT and composite draws)
+// ---------------------------------------------------------------------------
+
+// One stream of draws: xoshiro256** state, its CDF table (offset into the
+// concatenated tables), the number of draws and where they go.
+struct DrawJob {
+	uint64_t s[4];
+	uint64_t cdf_off, len, ndraw, out_off;
+};
+
+constexpr uint32_t kDrawChunk = 1024;   // uniforms staged in LDS per round
+
+__device__ __forceinline__ uint64_t rotl64(uint64_t x, int k)
+{
+	return (x << k) | (x >> (64 - k));
+}
+
+// Index of the cumulative table y[0 .. len) that the uniform u selects:
+// the binary descent over halving strides, advancing while y[i + d] <= u,
+// then the linear walk while y[i] <= u short of the last entry
+// (lib/prob.c:67-78; the same comparisons, so ties resolve the same way).
+__device__ __forceinline__ uint64_t cdf_pick(const double *__restrict__ y,
+	uint64_t len, double u)
+{
+	uint64_t i = 0;
+	for (uint64_t d = len >> 1; d > 0; d >>= 1)
+		if (y[i + d] <= u)
+			i += d;
+	while (i + 1 < len && y[i] <= u)
+		i++;
+	return i;
+}
+
+// One 64-lane workgroup per stream.  The generator is one sequential
+// stream, so every lane steps the same (wave-uniform, scalar-unit) state and
+// lane 0 stages each uniform (x >> 11) * 2^-53 in LDS
+// (lib/xoshiro256ss.c:72-85, include/xoshiro256ss.h:58-59); the lanes then
+// invert the table for 64 draws at a time.
+__global__ __launch_bounds__(64) void k_cdf_draw(DrawJob *__restrict__ jobs,
+	const double *__restrict__ cdf, uint32_t *__restrict__ out)
+{
+	__shared__ double u[kDrawChunk];
+	const uint32_t lane = threadIdx.x;
+	DrawJob *J = jobs + blockIdx.x;
+	uint64_t s0 = J->s[0], s1 = J->s[1], s2 = J->s[2], s3 = J->s[3];
+	const uint64_t len = J->len, ndraw = J->ndraw, o = J->out_off;
+	const double *y = cdf + J->cdf_off;
+	for (uint64_t b = 0; b < ndraw; b += kDrawChunk) {
+		const uint32_t n = (uint32_t)(ndraw - b < kDrawChunk ?
+			ndraw - b : kDrawChunk);
+		for (uint32_t d = 0; d < n; d++) {
+			const uint64_t r = rotl64(s1 * 5, 7) * 9;
+			const uint64_t t = s1 << 17;
+			s2 ^= s0;
+			s3 ^= s1;
+			s1 ^= s2;
+			s0 ^= s3;
+			s2 ^= t;
+			s3 = rotl64(s3, 45);
+			if (lane == 0)
+				u[d] = (double)(r >> 11) * 0x1.0p-53;
+		}
+		__syncthreads();
+		for (uint32_t d = lane; d < n; d += 64)
+			out[o + b + d] = (uint32_t)cdf_pick(y, len, u[d]);
+		__syncthreads();
+	}
+	if (lane == 0) {
+		J->s[0] = s0;
+		J->s[1] = s1;
+		J->s[2] = s2;
+		J->s[3] = s3;
+	}
 }
 
 __device__ __forceinline__ uint64_t taper_idx(uint64_t v, uint32_t n_sites)
@@ -1317,17 +1471,42 @@ constexpr uint32_t kBatchHalfTiles = (1u << 12) | (1u << 13);
 
 bool g_pipe_init = false;
 uint32_t g_pipe_mask = 0;   // bit TB set: persistent pipelined form
+// Persistent half-tile form with prefetch (k_rot_fused_hpipe), per tile
+// width: amplitudes of each coset prefetched (0: off) and workgroups per
+// compute unit.
+uint32_t g_hpipe_pf[PH2HIP_TILE_MAX + 1] = {};
+uint32_t g_hpipe_wpc[PH2HIP_TILE_MAX + 1] = {};
 bool g_direct = true;       // first/last groups straight to and from HBM
+
+// Kernel forms from the environment, once: PHASE2_HIP_PIPE (mask of tile
+// widths in the pipelined whole-tile form), PHASE2_HIP_PREFETCH12 and
+// PHASE2_HIP_PREFETCH13 ("amplitudes:workgroups per CU" of the prefetching
+// half-tile form, amplitudes 8 or 16).
+void pipe_init()
+{
+	if (g_pipe_init)
+		return;
+	const char *e = getenv("PHASE2_HIP_PIPE");
+	g_pipe_mask = e ? (uint32_t)strtoul(e, nullptr, 0) : 0;
+	for (uint32_t tb = 12; tb <= 13; tb++) {
+		char name[40];
+		snprintf(name, sizeof name, "PHASE2_HIP_PREFETCH%u", tb);
+		const char *f = getenv(name);
+		unsigned pf = 0, wpc = 0;
+		if (f && sscanf(f, "%u:%u", &pf, &wpc) == 2 &&
+		    (pf == 8 || pf == 16) && wpc >= 1 && wpc <= 16) {
+			g_hpipe_pf[tb] = pf;
+			g_hpipe_wpc[tb] = wpc;
+		}
+	}
+	g_pipe_init = true;
+}
 
 template <int TB>
 int launch_fused(double2 *a, uint32_t nqb, const FusedPass &P,
 	uint64_t xunion, uint64_t zunion)
 {
-	if (!g_pipe_init) {
-		const char *e = getenv("PHASE2_HIP_PIPE");
-		g_pipe_mask = e ? (uint32_t)strtoul(e, nullptr, 0) : 0;
-		g_pipe_init = true;
-	}
+	pipe_init();
 	const uint64_t ntiles = UINT64_C(1) << (nqb - TB);
 	hipEvent_t stop;
 	ph2hip_launch_rec meta{};
@@ -1353,6 +1532,25 @@ int launch_fused(double2 *a, uint32_t nqb, const FusedPass &P,
 		const uint64_t gx = ntiles < kMaxGrid ? ntiles : kMaxGrid;
 		const uint64_t gy = (ntiles + gx - 1) / gx;
 		if constexpr ((kHalfTiles >> TB & 1) != 0) {
+			const uint32_t pf = g_hpipe_pf[TB];
+			uint64_t pg = (uint64_t)g_hpipe_wpc[TB] * g_ncu;
+			pg = pg < ntiles ? pg : ntiles;
+			if (pg >= 8)
+				pg &= ~UINT64_C(7);   // whole rounds over the XCDs
+			if ((P.dflags & 4u) && pf && pg) {
+				constexpr int D = group_depth(TB);
+				const dim3 blk((1u << TB) >> D);
+				if (pf == 16)
+					hipLaunchKernelGGL((k_rot_fused_hpipe<TB, D, 16>),
+						dim3((uint32_t)pg), blk, 0, g_stream, a, P,
+						ntiles);
+				else
+					hipLaunchKernelGGL((k_rot_fused_hpipe<TB, D, 8>),
+						dim3((uint32_t)pg), blk, 0, g_stream, a, P,
+						ntiles);
+				HIPCHK(hipGetLastError());
+				return prof_end(stop);
+			}
 			if (P.dflags & 4u) {
 				hipLaunchKernelGGL(
 					(k_rot_fused_half<TB, group_depth(TB)>),
@@ -1745,6 +1943,288 @@ int launch_batch(double2 *amp, const FusedPass *descs, const BatchItem *items,
 	return 0;
 }
 
+// Replay of repeated fused sweeps on narrow registers.  A Trotter step
+// applies the same circ_cache block (same terms, angles and schedule) to
+// the same register every step; below ~20 qubits its passes take a few
+// microseconds each and launching them one by one costs about as much.
+// The first time a call is seen it is remembered (a copy of its
+// arguments); the second time its launches are captured into a hipGraph
+// (kernel nodes only: the pass descriptors are kernel arguments); from
+// then on the graph is replayed.  Calls match only if every argument is
+// equal, compared in full, so a replay launches exactly what the direct
+// path would.
+constexpr int kReplaySlots = 32;
+
+struct ReplaySlot {
+	uint64_t key, used;
+	void *amp;
+	uint32_t nqb;
+	size_t n, npass;
+	struct ph2hip_rot *rot;
+	uint32_t *sched;            // pass_len, tile_bits, low_bits
+	hipGraph_t graph;
+	hipGraphExec_t exec;
+};
+
+ReplaySlot g_replay[kReplaySlots];
+uint64_t g_replay_clock = 0, g_replay_hits = 0;
+bool g_replay_init = false;
+uint32_t g_replay_max_qb = 20;
+
+uint64_t fnv_bytes(uint64_t h, const void *p, size_t n)
+{
+	const unsigned char *b = static_cast<const unsigned char *>(p);
+	for (size_t i = 0; i < n; i++)
+		h = (h ^ b[i]) * UINT64_C(0x100000001b3);
+	return h;
+}
+
+void replay_clear(ReplaySlot &r)
+{
+	if (r.exec)
+		(void)hipGraphExecDestroy(r.exec);
+	if (r.graph)
+		(void)hipGraphDestroy(r.graph);
+	free(r.rot);
+	free(r.sched);
+	r = ReplaySlot{};
+}
+
+bool replay_same(const ReplaySlot &r, uint64_t key, void *amp, uint32_t nqb,
+	const struct ph2hip_rot *rot, size_t n, const uint32_t *const sch[3],
+	size_t npass)
+{
+	if (!r.rot || r.key != key || r.amp != amp || r.nqb != nqb ||
+	    r.n != n || r.npass != npass ||
+	    memcmp(r.rot, rot, sizeof *rot * n) != 0)
+		return false;
+	for (int a = 0; a < 3; a++)
+		if (memcmp(r.sched + a * npass, sch[a],
+			    sizeof(uint32_t) * npass) != 0)
+			return false;
+	return true;
+}
+
+// The slot holding this call, or a fresh one (least recently used)
+// remembering it; *seen tells which.  nullptr if out of host memory.
+ReplaySlot *replay_slot(void *amp, uint32_t nqb, const struct ph2hip_rot *rot,
+	size_t n, const uint32_t *const sch[3], size_t npass, bool *seen)
+{
+	uint64_t key = fnv_bytes(UINT64_C(0xcbf29ce484222325), rot,
+		sizeof *rot * n);
+	for (int a = 0; a < 3; a++)
+		key = fnv_bytes(key, sch[a], sizeof(uint32_t) * npass);
+	ReplaySlot *old = &g_replay[0];
+	for (ReplaySlot &r : g_replay) {
+		if (replay_same(r, key, amp, nqb, rot, n, sch, npass)) {
+			r.used = ++g_replay_clock;
+			*seen = true;
+			return &r;
+		}
+		if (r.used < old->used)
+			old = &r;
+	}
+	*seen = false;
+	replay_clear(*old);
+	old->rot = static_cast<struct ph2hip_rot *>(malloc(sizeof *rot * n));
+	old->sched = static_cast<uint32_t *>(malloc(sizeof(uint32_t) * 3 *
+		npass));
+	if (!old->rot || !old->sched) {
+		replay_clear(*old);
+		return nullptr;
+	}
+	memcpy(old->rot, rot, sizeof *rot * n);
+	for (int a = 0; a < 3; a++)
+		memcpy(old->sched + a * npass, sch[a], sizeof(uint32_t) * npass);
+	old->key = key;
+	old->amp = amp;
+	old->nqb = nqb;
+	old->n = n;
+	old->npass = npass;
+	old->used = ++g_replay_clock;
+	return old;
+}
+
+} // namespace
+
+struct ph2hip_batch_plan {
+	FusedPass *desc;      // descriptors, job-major
+	BatchItem *items;     // work items, launch-major
+	size_t ndesc, maxp;
+	size_t nlaunch;       // one or two per pass index (half-tile, whole)
+	size_t *lstart;       // launch l: items [lstart[l], lstart[l+1])
+	uint64_t *ltiles;     // tiles of launch l
+	uint8_t *lhalf;       // launch l in the half-tile form
+	uint32_t *lpass;      // pass index of launch l
+	uint32_t tile_bits;
+	uint64_t namp_total;
+};
+
+namespace {
+
+// The batched rounds' graph, built once and replayed: two kernel nodes
+// per pass index (the half-tile form, then the whole-tile form; one node
+// at tile widths without a half-tile form), chained in that order.  Each
+// round sets every node's arguments (its work items, or none: a node
+// without work exits at once) and launches the graph; it is rebuilt only
+// when a round needs more pass indices or another tile width.  The node
+// arguments change only after the previous replay has finished (an event
+// recorded behind it).
+struct RoundGraph {
+	uint32_t tile_bits = 0, per = 0;
+	size_t npass = 0;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	hipGraphNode_t *nodes = nullptr;
+	hipEvent_t done = nullptr;
+	bool pending = false;
+};
+RoundGraph g_rg;
+uint64_t g_rg_builds = 0, g_rg_replays = 0;
+
+void rg_clear()
+{
+	if (g_rg.pending && g_rg.done)
+		(void)hipEventSynchronize(g_rg.done);
+	if (g_rg.exec)
+		(void)hipGraphExecDestroy(g_rg.exec);
+	if (g_rg.graph)
+		(void)hipGraphDestroy(g_rg.graph);
+	if (g_rg.done)
+		(void)hipEventDestroy(g_rg.done);
+	free(g_rg.nodes);
+	g_rg = RoundGraph{};
+}
+
+// Arguments of one batched node (k_rot_fused_batch's parameters).
+struct BatchArgs {
+	double2 *amp;
+	const FusedPass *descs;
+	const BatchItem *items;
+	uint32_t nitems;
+	uint64_t ntiles;
+	void *ptr[5];
+};
+
+template <int TB>
+void batch_node(hipKernelNodeParams &kp, BatchArgs &ba, bool half)
+{
+	constexpr int D = group_depth(TB);
+	ba.ptr[0] = &ba.amp;
+	ba.ptr[1] = &ba.descs;
+	ba.ptr[2] = &ba.items;
+	ba.ptr[3] = &ba.nitems;
+	ba.ptr[4] = &ba.ntiles;
+	const uint64_t nt = ba.ntiles;
+	const uint64_t gx = nt == 0 ? 1 : nt < kMaxGrid ? nt : kMaxGrid;
+	const uint64_t gy = nt == 0 ? 1 : (nt + gx - 1) / gx;
+	kp = hipKernelNodeParams{};
+	if constexpr ((kBatchHalfTiles >> TB & 1) != 0) {
+		if (half) {
+			kp.func = reinterpret_cast<void *>(
+				&k_rot_fused_batch<TB, D, true>);
+			kp.blockDim = dim3((1u << TB) >> D);
+		}
+	}
+	if (!kp.func) {
+		kp.func = reinterpret_cast<void *>(&k_rot_fused_batch<TB, D>);
+		kp.blockDim = dim3(kBlock);
+	}
+	kp.gridDim = dim3((uint32_t)gx, (uint32_t)gy);
+	kp.sharedMemBytes = 0;
+	kp.kernelParams = ba.ptr;
+	kp.extra = nullptr;
+}
+
+void batch_node_tb(uint32_t tb, hipKernelNodeParams &kp, BatchArgs &ba,
+	bool half)
+{
+	switch (tb) {
+	case 9: batch_node<9>(kp, ba, half); break;
+	case 10: batch_node<10>(kp, ba, half); break;
+	case 11: batch_node<11>(kp, ba, half); break;
+	case 12: batch_node<12>(kp, ba, half); break;
+	default: batch_node<13>(kp, ba, half); break;
+	}
+}
+
+int rg_build(uint32_t tb, size_t npass, double2 *amp)
+{
+	rg_clear();
+	g_rg.tile_bits = tb;
+	g_rg.per = (kBatchHalfTiles >> tb & 1) ? 2 : 1;
+	g_rg.npass = npass;
+	const size_t nn = npass * g_rg.per;
+	g_rg.nodes = static_cast<hipGraphNode_t *>(calloc(nn ? nn : 1,
+		sizeof(hipGraphNode_t)));
+	if (!g_rg.nodes)
+		return fail("round graph: out of host memory");
+	HIPCHK(hipEventCreateWithFlags(&g_rg.done, hipEventDisableTiming));
+	HIPCHK(hipGraphCreate(&g_rg.graph, 0));
+	for (size_t i = 0; i < nn; i++) {
+		BatchArgs ba{ amp, nullptr, nullptr, 0, 0, {} };
+		hipKernelNodeParams kp;
+		batch_node_tb(tb, kp, ba, g_rg.per == 2 && i % 2 == 0);
+		HIPCHK(hipGraphAddKernelNode(&g_rg.nodes[i], g_rg.graph,
+			i ? &g_rg.nodes[i - 1] : nullptr, i ? 1 : 0, &kp));
+	}
+	HIPCHK(hipGraphInstantiate(&g_rg.exec, g_rg.graph, nullptr, nullptr, 0));
+	g_rg_builds++;
+	return 0;
+}
+
+// One round of a plan through the round graph (descriptors and items
+// already uploaded to g_bdesc / g_bitem).
+int rg_launch(const struct ph2hip_batch_plan *pl, double2 *amp)
+{
+	const uint32_t tb = pl->tile_bits;
+	if (!g_rg.exec || g_rg.tile_bits != tb || g_rg.npass < pl->maxp) {
+		const size_t np = g_rg.tile_bits == tb && g_rg.npass > pl->maxp ?
+			g_rg.npass : pl->maxp;
+		if (rg_build(tb, np, amp))
+			return -1;
+	} else if (g_rg.pending) {
+		HIPCHK(hipEventSynchronize(g_rg.done));
+	}
+	g_rg.pending = false;
+	// launch of each node: l for (pass p, form), or none
+	const size_t nn = g_rg.npass * g_rg.per;
+	long *lof = static_cast<long *>(malloc(sizeof(long) * (nn ? nn : 1)));
+	if (!lof)
+		return fail("round graph: out of host memory");
+	for (size_t i = 0; i < nn; i++)
+		lof[i] = -1;
+	for (size_t l = 0; l < pl->nlaunch; l++) {
+		const size_t slot = g_rg.per == 2 ?
+			2 * pl->lpass[l] + (pl->lhalf[l] ? 0 : 1) : pl->lpass[l];
+		lof[slot] = (long)l;
+	}
+	int rc = 0;
+	for (size_t i = 0; i < nn && !rc; i++) {
+		BatchArgs ba{ amp, static_cast<const FusedPass *>(g_bdesc),
+			static_cast<const BatchItem *>(g_bitem), 0, 0, {} };
+		if (lof[i] >= 0) {
+			const size_t l = (size_t)lof[i];
+			ba.items += pl->lstart[l];
+			ba.nitems = (uint32_t)(pl->lstart[l + 1] - pl->lstart[l]);
+			ba.ntiles = pl->ltiles[l];
+		}
+		hipKernelNodeParams kp;
+		batch_node_tb(tb, kp, ba, g_rg.per == 2 && i % 2 == 0);
+		if (hipGraphExecKernelNodeSetParams(g_rg.exec, g_rg.nodes[i], &kp)
+			!= hipSuccess)
+			rc = fail("round graph: node %zu arguments", i);
+	}
+	free(lof);
+	if (rc)
+		return rc;
+	HIPCHK(hipGraphLaunch(g_rg.exec, g_stream));
+	HIPCHK(hipEventRecord(g_rg.done, g_stream));
+	g_rg.pending = true;
+	g_rg_replays++;
+	return 0;
+}
+
 } // namespace
 
 // ---------------------------------------------------------------------------
@@ -1794,6 +2274,9 @@ void ph2hip_shutdown(void)
 	if (!g_stream)
 		return;
 	(void)hipStreamSynchronize(g_stream);
+	for (ReplaySlot &r : g_replay)
+		replay_clear(r);
+	rg_clear();
 	if (g_red)
 		(void)hipFree(g_red);
 	g_red = nullptr;
@@ -1957,6 +2440,26 @@ int ph2hip_rotate_fused(void *amp, uint32_t nqb, const struct ph2hip_rot *rot,
 			return fail("rotate_fused: term %zu flips qubit >= %u", k,
 				nqb);
 
+	if (!g_replay_init) {
+		const char *e = getenv("PHASE2_HIP_REPLAY_QB");
+		if (e)
+			g_replay_max_qb = (uint32_t)strtoul(e, nullptr, 0);
+		g_replay_init = true;
+	}
+	ReplaySlot *slot = nullptr;
+	if (nqb <= g_replay_max_qb && !g_prof && n > 0) {
+		const uint32_t *const sch[3] = { pass_len, tile_bits, low_bits };
+		bool seen = false;
+		slot = replay_slot(amp, nqb, rot, n, sch, npass, &seen);
+		if (slot && slot->exec) {
+			HIPCHK(hipGraphLaunch(slot->exec, g_stream));
+			g_replay_hits++;
+			return 0;
+		}
+		if (!seen)
+			slot = nullptr;   // first sight: launch directly
+	}
+
 	// Build every pass first, so nothing is applied if one does not fit.
 	FusedPass *passes = (FusedPass *)calloc(npass ? npass : 1,
 		sizeof(FusedPass));
@@ -1973,6 +2476,10 @@ int ph2hip_rotate_fused(void *amp, uint32_t nqb, const struct ph2hip_rot *rot,
 	}
 	double2 *a = (double2 *)amp;
 	int rc = 0;
+	// second sight of a call on a narrow register: capture its launches
+	if (slot && hipStreamBeginCapture(g_stream,
+			    hipStreamCaptureModeThreadLocal) != hipSuccess)
+		slot = nullptr;
 	k0 = 0;
 	for (size_t p = 0; p < npass && rc == 0; p++) {
 		uint64_t xu = 0, zu = 0;
@@ -1992,7 +2499,41 @@ int ph2hip_rotate_fused(void *amp, uint32_t nqb, const struct ph2hip_rot *rot,
 		}
 	}
 	free(passes);
+	if (slot) {
+		hipGraph_t graph = nullptr;
+		const hipError_t e = hipStreamEndCapture(g_stream, &graph);
+		if (!rc && e != hipSuccess)
+			rc = fail("rotate_fused: end capture: %s",
+				hipGetErrorString(e));
+		if (!rc && hipGraphInstantiate(&slot->exec, graph, nullptr,
+			    nullptr, 0) != hipSuccess) {
+			slot->exec = nullptr;
+			rc = fail("rotate_fused: graph instantiate");
+		}
+		slot->graph = graph;
+		if (!rc && hipGraphLaunch(slot->exec, g_stream) != hipSuccess)
+			rc = fail("rotate_fused: graph launch");
+		if (rc)
+			replay_clear(*slot);
+	}
 	return rc;
+}
+
+int ph2hip_set_replay(uint32_t max_qb)
+{
+	if (ensure_init())
+		return -1;
+	g_replay_init = true;
+	g_replay_max_qb = max_qb;
+	HIPCHK(hipStreamSynchronize(g_stream));
+	for (ReplaySlot &r : g_replay)
+		replay_clear(r);
+	return 0;
+}
+
+uint64_t ph2hip_replay_hits(void)
+{
+	return g_replay_hits;
 }
 
 int ph2hip_gather(const void *amp, uint32_t nqb, const uint64_t *idx,
@@ -2230,8 +2771,8 @@ int ph2hip_set_fused_form(uint32_t pipe_mask, uint64_t persistent_wgs)
 {
 	if (ensure_init())
 		return -1;
+	pipe_init();
 	g_pipe_mask = pipe_mask;
-	g_pipe_init = true;
 	if (persistent_wgs)
 		g_persist_wgs = persistent_wgs;
 	return 0;
@@ -2240,6 +2781,26 @@ int ph2hip_set_fused_form(uint32_t pipe_mask, uint64_t persistent_wgs)
 int ph2hip_set_fused_direct(int on)
 {
 	g_direct = on != 0;
+	return 0;
+}
+
+int ph2hip_set_fused_prefetch(uint32_t tile_bits, uint32_t amps,
+	uint32_t wgs_per_cu)
+{
+	if (ensure_init())
+		return -1;
+	if (tile_bits != 12 && tile_bits != 13)
+		return fail("set_fused_prefetch: tile %u has no half-tile form",
+			tile_bits);
+	if (amps != 0 && amps != 8 && amps != 16)
+		return fail("set_fused_prefetch: %u amplitudes (0, 8 or 16)",
+			amps);
+	if (amps && (wgs_per_cu == 0 || wgs_per_cu > 16))
+		return fail("set_fused_prefetch: %u workgroups per CU",
+			wgs_per_cu);
+	pipe_init();
+	g_hpipe_pf[tile_bits] = amps;
+	g_hpipe_wpc[tile_bits] = wgs_per_cu;
 	return 0;
 }
 
@@ -2277,6 +2838,81 @@ int ph2hip_elapsed_ms(int a, int b, double *ms)
 	HIPCHK(hipEventElapsedTime(&f, g_marks[a], g_marks[b]));
 	*ms = f;
 	return 0;
+}
+
+int ph2hip_sample_cdf(size_t nstream, const double *const *cdf,
+	const size_t *len, uint64_t *state, const size_t *ndraw, uint32_t *out)
+{
+	if (ensure_init())
+		return -1;
+	if (nstream == 0)
+		return 0;
+	size_t ntab = 0, nout = 0;
+	for (size_t k = 0; k < nstream; k++) {
+		if (len[k] == 0 || len[k] > UINT32_MAX || !cdf[k])
+			return fail("sample_cdf: stream %zu: table of %zu "
+				"entries", k, len[k]);
+		if (!(state[4 * k] | state[4 * k + 1] | state[4 * k + 2] |
+			    state[4 * k + 3]))
+			return fail("sample_cdf: stream %zu: zero state", k);
+		ntab += len[k];
+		nout += ndraw[k];
+	}
+	DrawJob *hj = static_cast<DrawJob *>(malloc(sizeof(DrawJob) *
+		nstream));
+	if (!hj)
+		return fail("sample_cdf: out of host memory");
+	size_t to = 0, oo = 0;
+	for (size_t k = 0; k < nstream; k++) {
+		for (int i = 0; i < 4; i++)
+			hj[k].s[i] = state[4 * k + i];
+		hj[k].cdf_off = to;
+		hj[k].len = len[k];
+		hj[k].ndraw = ndraw[k];
+		hj[k].out_off = oo;
+		to += len[k];
+		oo += ndraw[k];
+	}
+	double *dtab = nullptr;
+	DrawJob *dj = nullptr;
+	uint32_t *dout = nullptr;
+	int rc = 0;
+	if (hipMalloc(&dtab, sizeof(double) * ntab) != hipSuccess ||
+	    hipMalloc(&dj, sizeof(DrawJob) * nstream) != hipSuccess ||
+	    hipMalloc(&dout, sizeof(uint32_t) * (nout ? nout : 1)) !=
+		    hipSuccess)
+		rc = fail("sample_cdf: hipMalloc");
+	for (size_t k = 0; k < nstream && !rc; k++)
+		if (hipMemcpyAsync(dtab + hj[k].cdf_off, cdf[k],
+			    sizeof(double) * len[k], hipMemcpyHostToDevice,
+			    g_stream) != hipSuccess)
+			rc = fail("sample_cdf: copy of table %zu", k);
+	if (!rc && hipMemcpyAsync(dj, hj, sizeof(DrawJob) * nstream,
+			   hipMemcpyHostToDevice, g_stream) != hipSuccess)
+		rc = fail("sample_cdf: copy of the streams");
+	if (!rc) {
+		hipLaunchKernelGGL(k_cdf_draw, dim3((uint32_t)nstream), dim3(64),
+			0, g_stream, dj, dtab, dout);
+		if (hipGetLastError() != hipSuccess)
+			rc = fail("sample_cdf: launch");
+	}
+	if (!rc && nout && hipMemcpyAsync(out, dout, sizeof(uint32_t) * nout,
+			   hipMemcpyDeviceToHost, g_stream) != hipSuccess)
+		rc = fail("sample_cdf: copy out");
+	if (!rc && hipMemcpyAsync(hj, dj, sizeof(DrawJob) * nstream,
+			   hipMemcpyDeviceToHost, g_stream) != hipSuccess)
+		rc = fail("sample_cdf: copy of the states");
+	if (hipStreamSynchronize(g_stream) != hipSuccess && !rc)
+		rc = fail("sample_cdf: sync");
+	if (!rc)
+		for (size_t k = 0; k < nstream; k++)
+			for (int i = 0; i < 4; i++)
+				state[4 * k + i] = hj[k].s[i];
+	(void)hipFree(dtab);
+	(void)hipFree(dj);
+	(void)hipFree(dout);
+	free(hj);
+	return rc;
 }
 
 int ph2hip_norm2(const void *amp, uint32_t nqb, double *out)
@@ -2330,33 +2966,17 @@ int ph2hip_count_unequal(const void *a, const void *b, uint64_t namp,
 	return 0;
 }
 
-struct ph2hip_batch_plan {
-	FusedPass *desc;      // descriptors, job-major
-	BatchItem *items;     // work items, launch-major
-	size_t ndesc, maxp;
-	size_t nlaunch;       // one or two per pass index (half-tile, whole)
-	size_t *lstart;       // launch l: items [lstart[l], lstart[l+1])
-	uint64_t *ltiles;     // tiles of launch l
-	uint8_t *lhalf;       // launch l in the half-tile form
-	uint32_t tile_bits;
-	uint64_t namp_total;
-	hipGraph_t graph;
-	hipGraphExec_t exec;
-};
 
 void ph2hip_batch_plan_free(struct ph2hip_batch_plan *pl)
 {
 	if (!pl)
 		return;
-	if (pl->exec)
-		(void)hipGraphExecDestroy(pl->exec);
-	if (pl->graph)
-		(void)hipGraphDestroy(pl->graph);
 	free(pl->desc);
 	free(pl->items);
 	free(pl->lstart);
 	free(pl->ltiles);
 	free(pl->lhalf);
+	free(pl->lpass);
 	free(pl);
 }
 
@@ -2406,10 +3026,11 @@ int ph2hip_batch_plan_build(const struct ph2hip_batch_job *jobs, size_t njobs,
 	pl->lstart = (size_t *)calloc(2 * maxp + 1, sizeof(size_t));
 	pl->ltiles = (uint64_t *)calloc(2 * maxp, sizeof(uint64_t));
 	pl->lhalf = (uint8_t *)calloc(2 * maxp, 1);
+	pl->lpass = (uint32_t *)calloc(2 * maxp, sizeof(uint32_t));
 	size_t *first = (size_t *)malloc(sizeof(size_t) * (njobs ? njobs : 1));
 	int rc = 0;
 	if (!pl->desc || !pl->items || !pl->lstart || !pl->ltiles ||
-	    !pl->lhalf || !first) {
+	    !pl->lhalf || !pl->lpass || !first) {
 		rc = fail("rotate_batched: out of host memory");
 	} else {
 		// descriptors, job-major
@@ -2458,6 +3079,7 @@ int ph2hip_batch_plan_build(const struct ph2hip_batch_job *jobs, size_t njobs,
 				pl->lstart[l] = it0;
 				pl->ltiles[l] = t0;
 				pl->lhalf[l] = (uint8_t)h;
+				pl->lpass[l] = (uint32_t)p;
 				l++;
 			}
 		}
@@ -2494,13 +3116,16 @@ int ph2hip_batch_plan_launch(void *amp, struct ph2hip_batch_plan *pl,
 	    hipMemcpyAsync(g_bitem, pl->items, sizeof(BatchItem) * ndesc,
 		    hipMemcpyHostToDevice, g_stream) != hipSuccess)
 		return fail("rotate_batched: upload");
-	// Profiling brackets every launch with events, which stays out of
-	// graph capture.
-	const bool graph = use_graph && !g_prof;
+	// Profiling brackets every launch with events, so it launches
+	// directly; otherwise the round graph runs the round.
+	if (use_graph && !g_prof) {
+		if (rg_launch(pl, (double2 *)amp))
+			return -1;
+		if (launches)
+			*launches = nlaunch;
+		return 0;
+	}
 	int rc = 0;
-	if (graph && hipStreamBeginCapture(g_stream,
-		    hipStreamCaptureModeThreadLocal) != hipSuccess)
-		return fail("rotate_batched: begin capture");
 	for (size_t l = 0; l < nlaunch && !rc; l++) {
 		const uint32_t ni = (uint32_t)(pl->lstart[l + 1] - pl->lstart[l]);
 		const BatchItem *it = (const BatchItem *)g_bitem + pl->lstart[l];
@@ -2511,7 +3136,7 @@ int ph2hip_batch_plan_launch(void *amp, struct ph2hip_batch_plan *pl,
 		meta.tile_bits = tile_bits;
 		meta.ngroups = ni;
 		meta.direct = hf ? 4u : 0u;
-		if (!graph && prof_begin(3, ni, pl->ltiles[l] << tile_bits, &stop,
+		if (prof_begin(3, ni, pl->ltiles[l] << tile_bits, &stop,
 			    &meta)) {
 			rc = -1;
 			break;
@@ -2525,23 +3150,18 @@ int ph2hip_batch_plan_launch(void *amp, struct ph2hip_batch_plan *pl,
 		case 12: rc = launch_batch<12>(a, dd, it, ni, nt, hf); break;
 		default: rc = launch_batch<13>(a, dd, it, ni, nt, hf); break;
 		}
-		if (!rc && !graph && prof_end(stop))
+		if (!rc && prof_end(stop))
 			rc = -1;
-	}
-	if (graph) {
-		const hipError_t e = hipStreamEndCapture(g_stream, &pl->graph);
-		if (!rc && e != hipSuccess)
-			rc = fail("rotate_batched: end capture: %s",
-				hipGetErrorString(e));
-		if (!rc && hipGraphInstantiate(&pl->exec, pl->graph, nullptr,
-			    nullptr, 0) != hipSuccess)
-			rc = fail("rotate_batched: graph instantiate");
-		if (!rc && hipGraphLaunch(pl->exec, g_stream) != hipSuccess)
-			rc = fail("rotate_batched: graph launch");
 	}
 	if (!rc && launches)
 		*launches = nlaunch;
 	return rc;
+}
+
+void ph2hip_round_graph_stats(uint64_t *builds, uint64_t *replays)
+{
+	*builds = g_rg_builds;
+	*replays = g_rg_replays;
 }
 
 int ph2hip_rotate_batched(void *amp, uint64_t namp_total,

@@ -117,7 +117,7 @@ static inline int topbit(uint64_t v)
  * low-bit count: the end just before each term that widens the span, the
  * longest feasible end, and the single term. */
 static size_t candidates(const struct ph2hip_rot *rot, size_t n,
-	uint32_t nqb, size_t cap)
+	uint32_t nqb, uint32_t low0, struct cand *out, size_t cap)
 {
 	size_t m = 0;
 	for (size_t i = 0; i < n; i++) {
@@ -130,9 +130,9 @@ static size_t candidates(const struct ph2hip_rot *rot, size_t n,
 				    (TILES[t - 1] < nqb ? TILES[t - 1] : nqb)))
 				continue;
 			/* one tile bit at least is left for the flip masks */
-			const uint32_t low = LM.low + dl < tb ? LM.low + dl : tb - 1;
-			if (dl > 0 && low == (LM.low + dl - 1 < tb ?
-				    LM.low + dl - 1 : tb - 1))
+			const uint32_t low = low0 + dl < tb ? low0 + dl : tb - 1;
+			if (dl > 0 && low == (low0 + dl - 1 < tb ?
+				    low0 + dl - 1 : tb - 1))
 				continue;
 			const uint64_t lowmask = (UINT64_C(1) << low) - 1;
 			const uint32_t gmax = tb - low;
@@ -148,7 +148,7 @@ static size_t candidates(const struct ph2hip_rot *rot, size_t n,
 					if (g == gmax)
 						break;
 					if (j > i + 1 && m < cap)
-						LM.cand[m++] = (struct cand){ (uint32_t)i,
+						out[m++] = (struct cand){ (uint32_t)i,
 							(uint32_t)j, tb, low };
 					const int p = topbit(v);
 					for (uint32_t b = 0; b < g; b++)
@@ -158,14 +158,55 @@ static size_t candidates(const struct ph2hip_rot *rot, size_t n,
 				}
 			}
 			if (j > i + 1 && m < cap)
-				LM.cand[m++] = (struct cand){ (uint32_t)i, (uint32_t)j,
+				out[m++] = (struct cand){ (uint32_t)i, (uint32_t)j,
 					tb, low };
 			if (m < cap)
-				LM.cand[m++] = (struct cand){ (uint32_t)i,
+				out[m++] = (struct cand){ (uint32_t)i,
 					(uint32_t)(i + 1), tb, low };
 		}
 	}
 	return m;
+}
+
+/* A candidate the scheduler considers: it fits its tile, and it is not a
+ * multi-group tile-13 pass without the half-tile form (such a pass runs
+ * one 128 KiB workgroup per CU; random cuts, the training data, never
+ * produce one, so its prediction would be a guess; every start keeps its
+ * tile-12 candidates).  shape gets ph2hip_pass_shape's four entries. */
+static int keep_candidate(const struct ph2hip_rot *rot, struct cand cd,
+	uint32_t *shape)
+{
+	if (ph2hip_pass_shape(rot + cd.i, cd.j - cd.i, cd.tb, cd.low,
+		    shape) < 0)
+		return 0;
+	return !(cd.tb == 13 && shape[2] > 1 && !(shape[3] & 4u));
+}
+
+long ph2_learned_candidates(const struct ph2hip_rot *rot, size_t n,
+	uint32_t nqb, uint32_t low_bits, uint32_t *out, size_t cap)
+{
+	const size_t m0 = n * 2 * NLOW * (MAXLEN / 4 + 3) + 16;
+	struct cand *c = malloc(sizeof *c * m0);
+	if (!c)
+		return -1;
+	const size_t m = candidates(rot, n, nqb, low_bits, c, m0);
+	size_t k = 0;
+	for (size_t q = 0; q < m; q++) {
+		uint32_t shape[4];
+		if (!keep_candidate(rot, c[q], shape))
+			continue;
+		if (k == cap) {
+			free(c);
+			return -1;
+		}
+		out[4 * k] = c[q].i;
+		out[4 * k + 1] = c[q].j;
+		out[4 * k + 2] = c[q].tb;
+		out[4 * k + 3] = c[q].low;
+		k++;
+	}
+	free(c);
+	return (long)k;
 }
 
 static double to_ms(float y, uint32_t nqb)
@@ -188,20 +229,13 @@ int ph2_learned_schedule(const struct ph2hip_rot *rot, size_t n, uint32_t nqb,
 	const size_t cap = n * 2 * NLOW * (MAXLEN / 4 + 3) + 16;
 	if (reserve(cap) < 0)
 		return -1;
-	size_t m = candidates(rot, n, nqb, cap);
+	size_t m = candidates(rot, n, nqb, LM.low, LM.cand, cap);
 	/* shapes and features; drop candidates that do not fit */
 	size_t k = 0;
 	for (size_t c = 0; c < m; c++) {
 		uint32_t shape[4];
 		const struct cand cd = LM.cand[c];
-		if (ph2hip_pass_shape(rot + cd.i, cd.j - cd.i, cd.tb, cd.low,
-			    shape) < 0)
-			continue;
-		/* a multi-group tile-13 pass without the half-tile form runs
-		 * one 128 KiB workgroup per CU; random cuts (the training data)
-		 * never produce one, so its prediction is a guess: leave it out
-		 * (every start keeps its tile-12 candidates) */
-		if (cd.tb == 13 && shape[2] > 1 && !(shape[3] & 4u))
+		if (!keep_candidate(rot, cd, shape))
 			continue;
 		LM.cand[k] = cd;
 		ph2_learned_features(rot + cd.i, cd.j - cd.i, cd.tb, shape,

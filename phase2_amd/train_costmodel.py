@@ -25,6 +25,7 @@ report.json and report_cmpsit.json (steps 3-4) into DIR.
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import glob
 import json
 import sys
@@ -187,6 +188,82 @@ def eval_n33(params, log=log):
     return res
 
 
+def time_sched(reg, xm, zm, phi, sched, reps=3):
+    """Wall time (least of reps) of one block under an explicit schedule."""
+    L = nat.lib()
+    L.ph2_reg_rotate_sched.restype = C.c_int
+    u32p = C.POINTER(C.c_uint32)
+    lens, tbs, lbs = (np.ascontiguousarray(a, np.uint32) for a in sched)
+    best = 1e300
+    for _ in range(reps):
+        nat.synchronize()
+        t0 = time.perf_counter()
+        if L.ph2_reg_rotate_sched(
+                C.c_void_p(reg.h), C.c_size_t(len(xm)),
+                xm.ctypes.data_as(C.POINTER(C.c_uint64)),
+                zm.ctypes.data_as(C.POINTER(C.c_uint64)),
+                phi.ctypes.data_as(C.POINTER(C.c_double)),
+                lens.ctypes.data_as(u32p), tbs.ctypes.data_as(u32p),
+                lbs.ctypes.data_as(u32p), C.c_size_t(len(lens))) != 0:
+            raise RuntimeError("ph2_reg_rotate_sched")
+        nat.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    return best
+
+
+def eval_dp(params, nqb=26, blocks=(3, 9, 15), log=log):
+    """The learned schedule against the best schedule over the same
+    candidate passes by their measured times: every candidate pass of a
+    block is timed on its own, a shortest path over cut points picks the
+    measured optimum, and the greedy cuts, the learned schedule and that
+    optimum are timed as whole blocks (least of three)."""
+    hm = synth.hamiltonian(nqb, 0, seed=4242, decay=True,
+                           n_total=1024 * (max(blocks) + 2))
+    xm, zm, phi = sorted_terms(hm)
+    reg = nat.Register(nqb, nat.FUSED, 12, 3)
+    reg.prep_multidet(np.array([(1 << (nqb // 2)) - 1], np.uint64),
+                      np.array([1.0 + 0j]))
+    cm.install(params, min_qb=nqb)
+    rows = []
+    try:
+        for b in blocks:
+            sl = slice(b * 1024, (b + 1) * 1024)
+            bx, bz, bp = (np.ascontiguousarray(a[sl]) for a in (xm, zm, phi))
+            cand = cm.candidates(bx, bz, nqb)
+            t0 = time.perf_counter()
+            ms = cm.measure_candidates(reg, bx, bz, bp, cand, reps=1)
+            log(f"dp: block {b}: {len(cand)} candidates timed in "
+                f"{time.perf_counter() - t0:.1f}s")
+            opt, opt_ms = cm.dp_schedule(cand, ms, len(bx))
+            lrn = cm.schedule(bx, bz, nqb)
+            # the learned schedule's passes at their measured times
+            key = {tuple(c): k for k, c in enumerate(cand.tolist())}
+            i, lrn_ms = 0, 0.0
+            for ln, tb, low in zip(*lrn):
+                lrn_ms += ms[key[(i, i + int(ln), int(tb), int(low))]]
+                i += int(ln)
+            row = dict(block=b, candidates=int(len(cand)),
+                       optimum_sum_ms=opt_ms, learned_sum_ms=float(lrn_ms),
+                       optimum_passes=int(len(opt[0])),
+                       learned_passes=int(len(lrn[0])))
+            for name, sch in (("greedy_12_3", cm.schedule(bx, bz, nqb,
+                                                           greedy=(12, 3))),
+                              ("greedy_13_3", cm.schedule(bx, bz, nqb,
+                                                           greedy=(13, 3))),
+                              ("learned", lrn), ("optimum", opt)):
+                row[name + "_s"] = time_sched(reg, bx, bz, bp, sch)
+            log(f"dp: {row}")
+            rows.append(row)
+    finally:
+        cm.uninstall()
+        reg.free()
+    tot = {k: float(sum(r[k] for r in rows)) for k in
+           ("greedy_12_3_s", "greedy_13_3_s", "learned_s", "optimum_s")}
+    return dict(nqb=nqb, blocks=list(blocks), rows=rows, totals=tot,
+                learned_vs_optimum=tot["optimum_s"] / tot["learned_s"],
+                learned_vs_greedy=tot["greedy_12_3_s"] / tot["learned_s"])
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
@@ -204,12 +281,19 @@ def main(argv=None):
                     help="train on this saved records.npy instead of "
                     "collecting new timings")
     ap.add_argument("--cmpsit-steps", type=int, default=1)
+    ap.add_argument("--eval-dp", action="store_true",
+                    help="also compare the learned schedule with the "
+                    "measured-optimal schedule over the same candidates")
     args = ap.parse_args(argv)
     out = Path(args.out)
     out.mkdir(parents=True, exist_ok=True)
     nat.init()
     if args.eval_only:
         params, _ = cm.load(args.eval_only)
+        if args.eval_dp:
+            res_d = eval_dp(params)
+            (out / "report_dp.json").write_text(json.dumps(res_d, indent=1))
+            log(json.dumps(res_d))
         res = eval_n33(params)
         (out / "report.json").write_text(json.dumps(res, indent=1))
         log(json.dumps(res))
@@ -280,6 +364,10 @@ def main(argv=None):
     res_c = eval_cmpsit(params, args.cmpsit_steps)
     (out / "report_cmpsit.json").write_text(json.dumps(res_c, indent=1))
     log(json.dumps(res_c))
+    if args.eval_dp:
+        res_d = eval_dp(params)
+        (out / "report_dp.json").write_text(json.dumps(res_d, indent=1))
+        log(json.dumps(res_d))
 
 
 if __name__ == "__main__":
