@@ -109,6 +109,8 @@ int ph2hip_set_fused_direct(int on);
  * CUDA backend has no fused pass). */
 #define PH2HIP_HALF_TILES ((1u << 12) | (1u << 13))
 int ph2hip_set_fused_half(uint32_t tile_mask);
+/* The current mask. */
+uint32_t ph2hip_fused_half_mask(void);
 
 /* Persistent form of the half-tile pass at tile width 12 or 13: each
  * workgroup walks tiles and loads the next tile's first-group cosets (amps

@@ -246,7 +246,8 @@ __global__ __launch_bounds__(kBlock) void k_rot_diag(
 struct alignas(16) FusedTerm {
 	uint64_t z;    // physical phase mask (per-tile parity)
 	double c, s;
-	uint32_t zt;   // phase mask in tile coordinates
+	uint32_t zt;   // phase mask in tile coordinates (bits 0..15) and the
+	               // per-register signs parity(w & ztp) (bits 16..31)
 	union {
 		struct {
 			uint8_t xtp;   // flip mask in group coordinates
@@ -414,6 +415,14 @@ __device__ __forceinline__ void diag_inplace(double &re, double &im,
 		: [c] "s"(c), [sb] "v"(sb), [sg] "s"(sg));
 }
 
+// High word of +-1.0 for coset register w: -1.0 if bit w of the term's
+// sign mask is set.  A bit test and a select on the scalar unit.
+__device__ __forceinline__ double reg_sign(uint32_t sgn, int w)
+{
+	return __hiloint2double((sgn >> w & 1u) ? (int)0xbff00000 :
+		0x3ff00000, 0);
+}
+
 // +-1.0 with the sign bit f (an exact multiplier)
 __device__ __forceinline__ double unit_sign(uint32_t f)
 {
@@ -429,7 +438,7 @@ __device__ __forceinline__ double unit_sign(uint32_t f)
 // sign flipped by par ^ bit1(y); sb already carries bpt ^ bit1(y), so per
 // pair only parity(w & ztp) (uniform) flips.
 template <int D, int XT, bool ODD>
-__device__ __forceinline__ void coset_term(v2d (&a)[1 << D], uint32_t ztp,
+__device__ __forceinline__ void coset_term(v2d (&a)[1 << D], uint32_t sgn,
 	double sb, double c)
 {
 	constexpr int H = 31 - __builtin_clz(XT);
@@ -440,8 +449,7 @@ __device__ __forceinline__ void coset_term(v2d (&a)[1 << D], uint32_t ztp,
 			yi = a[w ^ XT].y;
 		// per pair only the uniform sign (scalar unit) changes: one
 		// exact multiply by +-1.0 held in scalar registers
-		pair_inplace<ODD>(xr, xi, yr, yi, c, sb,
-			unit_sign(__popc(w & ztp) & 1u));
+		pair_inplace<ODD>(xr, xi, yr, yi, c, sb, reg_sign(sgn, w));
 		a[w].x = xr;
 		a[w].y = xi;
 		a[w ^ XT].x = yr;
@@ -455,16 +463,16 @@ __device__ __forceinline__ void coset_term(v2d (&a)[1 << D], uint32_t ztp,
 // registers once per term at the loop back-edge.
 template <int D, bool ODD, int XT = 1>
 __device__ __forceinline__ void coset_dispatch(v2d (&a)[1 << D],
-	uint32_t xtp, uint32_t ztp, double sb, double c)
+	uint32_t xtp, uint32_t sgn, double sb, double c)
 {
 	if constexpr (XT == (1 << D) - 1) {
-		coset_term<D, XT, ODD>(a, ztp, sb, c);
+		coset_term<D, XT, ODD>(a, sgn, sb, c);
 	} else {
 		if (xtp == XT) {
-			coset_term<D, XT, ODD>(a, ztp, sb, c);
+			coset_term<D, XT, ODD>(a, sgn, sb, c);
 			return;
 		}
-		coset_dispatch<D, ODD, XT + 1>(a, xtp, ztp, sb, c);
+		coset_dispatch<D, ODD, XT + 1>(a, xtp, sgn, sb, c);
 	}
 }
 
@@ -496,9 +504,9 @@ __device__ __forceinline__ void group_terms(v2d (&a)[1 << D],
 		const uint32_t tzt_n = P.t[kn].zt,
 			meta_n = P.t[kn].meta;
 		const uint32_t xtp = meta & 0xffu;
-		const uint32_t ztp = (meta >> 8) & 0xffu;
 		const uint32_t ty = (meta >> 16) & 0xffu;
-		const uint32_t bpt = (__popc(bu & tzt) ^
+		const uint32_t sgn = tzt >> 16;
+		const uint32_t bpt = (__popc(bu & (tzt & 0xffffu)) ^
 			__popcll(base & tz)) & 1u;
 		if (xtp == 0) {
 			// diagonal: a <- c a + s_j (i a), s_j = +-s
@@ -506,8 +514,7 @@ __device__ __forceinline__ void group_terms(v2d (&a)[1 << D],
 #pragma unroll
 			for (int w = 0; w < (1 << D); w++) {
 				double re = a[w].x, im = a[w].y;
-				diag_inplace(re, im, c, sb,
-					unit_sign(__popc(w & ztp) & 1u));
+				diag_inplace(re, im, c, sb, reg_sign(sgn, w));
 				a[w].x = re;
 				a[w].y = im;
 			}
@@ -515,10 +522,10 @@ __device__ __forceinline__ void group_terms(v2d (&a)[1 << D],
 			const double sb = flip_sign(ts,
 				bpt ^ ((ty >> 1) & 1u));
 			if (ty & 1)
-				coset_dispatch<D, true>(a, xtp, ztp, sb,
+				coset_dispatch<D, true>(a, xtp, sgn, sb,
 					c);
 			else
-				coset_dispatch<D, false>(a, xtp, ztp, sb,
+				coset_dispatch<D, false>(a, xtp, sgn, sb,
 					c);
 		}
 		tz = tz_n;
@@ -1837,6 +1844,14 @@ int build_pass_mode(FusedPass &P, const struct ph2hip_rot *r, size_t len,
 				return fail("rotate_fused: group basis");
 			P.t[q].xtp = (uint8_t)xp;
 			P.t[q].ztp = (uint8_t)zp;
+			// bit w of the top half: parity(w & ztp), the sign of
+			// coset register w's partner term (tile coordinates use
+			// at most the low 13 bits of zt)
+			uint32_t sgn = 0;
+			for (uint32_t w = 0; w < (1u << D); w++)
+				sgn |= (uint32_t)(__builtin_popcount(w & zp) & 1) <<
+					w;
+			P.t[q].zt = (P.t[q].zt & 0xffffu) | (sgn << 16);
 		}
 	}
 	P.ngroups = (uint8_t)ng;
@@ -2802,6 +2817,11 @@ int ph2hip_set_fused_prefetch(uint32_t tile_bits, uint32_t amps,
 	g_hpipe_pf[tile_bits] = amps;
 	g_hpipe_wpc[tile_bits] = wgs_per_cu;
 	return 0;
+}
+
+uint32_t ph2hip_fused_half_mask(void)
+{
+	return half_mask();
 }
 
 int ph2hip_set_fused_half(uint32_t tile_mask)

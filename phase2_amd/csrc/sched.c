@@ -7,6 +7,7 @@
 #include <stdlib.h>
 
 #include "phase2/qreg_hip.h"
+#include "phase2_hip.h"
 
 void ph2_sched_free(struct ph2_sched *s)
 {
@@ -55,6 +56,59 @@ static inline int topbit(uint64_t v)
 	return 63 - __builtin_clzll(v);
 }
 
+/* 1 if the pass rot[0..n) in tiles of tb runs in the half-tile form or
+ * needs no exchange (one micro-group), 0 if not, -1 if it does not fit. */
+static int half_or_single(const struct ph2hip_rot *rot, size_t n, uint32_t tb,
+	uint32_t low)
+{
+	uint32_t shape[4];
+	if (ph2hip_pass_shape(rot, n, tb, low, shape) < 0)
+		return -1;
+	return shape[2] <= 1 || (shape[3] & 4u) != 0;
+}
+
+/* Append the pass rot[0..len) at tile 13, cut into the longest prefixes
+ * that keep the half-tile form: without it a multi-group tile-13 pass
+ * runs one 128 KiB workgroup per CU, measured 2.8x the time of a
+ * half-tile pass over the same register. */
+static int push_split13(struct ph2_sched *out, const struct ph2hip_rot *rot,
+	uint32_t len, uint32_t low)
+{
+	while (len > 0) {
+		uint32_t lo = 1, hi = len;   /* longest good prefix in [lo, hi] */
+		const int whole = half_or_single(rot, len, 13, low);
+		if (whole < 0)
+			return -1;
+		if (whole) {
+			lo = len;
+		} else {
+			while (lo < hi) {
+				const uint32_t mid = (lo + hi + 1) / 2;
+				const int ok = half_or_single(rot, mid, 13, low);
+				if (ok < 0)
+					return -1;
+				if (ok)
+					lo = mid;
+				else
+					hi = mid - 1;
+			}
+		}
+		if (ph2_sched_push(out, lo, 13, low) < 0)
+			return -1;
+		rot += lo;
+		len -= lo;
+	}
+	return 0;
+}
+
+static int push(struct ph2_sched *out, const struct ph2hip_rot *rot,
+	uint32_t len, uint32_t tb, uint32_t low)
+{
+	if (tb == 13 && (ph2hip_fused_half_mask() >> 13 & 1))
+		return push_split13(out, rot, len, low);
+	return ph2_sched_push(out, len, tb, low);
+}
+
 int ph2_sched_greedy(const struct ph2hip_rot *rot, size_t n, uint32_t nqb,
 	uint32_t tile_bits, uint32_t low_bits, struct ph2_sched *out)
 {
@@ -79,7 +133,8 @@ int ph2_sched_greedy(const struct ph2hip_rot *rot, size_t n, uint32_t nqb,
 				v ^= basis[b];
 		const int grows = v != 0;
 		if (len == PH2HIP_PASS_MAX || (grows && g == gmax)) {
-			if (ph2_sched_push(out, len, tile_bits, low_bits) < 0)
+			if (push(out, rot + k - len, len, tile_bits, low_bits) <
+				0)
 				return -1;
 			g = 0;
 			len = 0;
@@ -96,7 +151,7 @@ int ph2_sched_greedy(const struct ph2hip_rot *rot, size_t n, uint32_t nqb,
 		}
 		len++;
 	}
-	if (len && ph2_sched_push(out, len, tile_bits, low_bits) < 0)
+	if (len && push(out, rot + n - len, len, tile_bits, low_bits) < 0)
 		return -1;
 	return 0;
 }

@@ -43,8 +43,17 @@ def lib():
     return ctypes.CDLL(str(LIB))
 
 
+# headers whose functions live in the simul.h5 library (libphase2_h5.so)
+H5_HEADERS = ("include/ph2run/data.h", "include/phase2_h5.h")
+H5LIB = ROOT / "phase2_amd" / "libphase2_h5.so"
+
+
 @pytest.mark.parametrize("header,name", declared())
 def test_symbol_exported(lib, header, name):
+    if header in H5_HEADERS:
+        if not H5LIB.is_file():
+            pytest.skip("libphase2_h5.so not built (no HDF5 headers)")
+        lib = ctypes.CDLL(str(H5LIB))
     assert hasattr(lib, name), f"{name} from {header} not exported"
 
 
