@@ -23,8 +23,9 @@ trott2_n26
     half-sweeps + coefficient-matrix overlap).
 qdrift_batch
     512 qDRIFT registers of 16..26 qubits (107 GiB in one device buffer),
-    depth 1000; a step is one sample of every register, run as batched
-    launches (one per pass index) captured in a hipGraph per round.
+    depth 1000; a step is one sample of every register: draws made on the
+    device before the first round, batched launches (one per pass index
+    and form) through one graph built once and replayed every round.
 cmpsit_n30
     composite integrator at 30 qubits, 60,000 terms, 2,000 deterministic +
     512 sampled rotations per half-step; a step is one cmpsit step.
@@ -430,6 +431,16 @@ def bench_trott_n16(args, nat, costmodel, schedule):
     out, _ = run(args.steps)
     nat.synchronize()
     elapsed = time.perf_counter() - t0
+    # the same run with the graph replay of repeated sweeps turned off
+    import ctypes
+    L = nat.lib()
+    L.ph2hip_set_replay.restype = ctypes.c_int
+    L.ph2hip_set_replay(ctypes.c_uint32(0))
+    t0 = time.perf_counter()
+    run(args.steps)
+    nat.synchronize()
+    no_replay = time.perf_counter() - t0
+    L.ph2hip_set_replay(ctypes.c_uint32(20))
     _, prof = _timed_driver(nat, run, args.steps, 0)
     kern, kp, tiles = _dominant(prof)
     nat.set_default(nat.PLAIN)
@@ -447,7 +458,9 @@ def bench_trott_n16(args, nat, costmodel, schedule):
         config={"workload": "trott1_n16_full", "nqb": 16, "terms": len(hm),
                 "steps": args.steps, "delta": w["delta"]},
         extra={"overlap_last": [float(out[-1].real), float(out[-1].imag)],
-               "launches_per_kernel": tiles},
+               "launches_per_kernel": tiles,
+               "ms_per_step_replay": 1e3 * elapsed / args.steps,
+               "ms_per_step_no_replay": 1e3 * no_replay / args.steps},
         cpu=lambda ranks: cpu_circ(
             1, synth.trott_n16()["hm"], w["det_idx"], w["det_cf"], ranks,
             f"trott n=16, {args.steps} steps of {len(hm)} terms",
@@ -537,10 +550,15 @@ def bench_qdrift_batch(args, nat, costmodel, schedule):
     if args.warmup:
         nat.batch_qdrift(warm, nat.BATCH_FUSED_BATCHED, args.tile, args.low)
     nat.synchronize()
+    import ctypes
+    gb, gr = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    nat.lib().ph2hip_round_graph_stats(ctypes.byref(gb), ctypes.byref(gr))
+    g0 = (gb.value, gr.value)
     t0 = time.perf_counter()
     _, st = nat.batch_qdrift(regs, nat.BATCH_FUSED_BATCHED, 0, args.low)
     nat.synchronize()
     elapsed = time.perf_counter() - t0
+    nat.lib().ph2hip_round_graph_stats(ctypes.byref(gb), ctypes.byref(gr))
     # kernel timing on a subset, without graph capture (profiling brackets
     # every launch with events)
     sub = [dict(r, samples=1) for r in regs[:64]]
@@ -572,6 +590,9 @@ def bench_qdrift_batch(args, nat, costmodel, schedule):
                "launches": st["launches"], "passes": st["passes"],
                "tile_bits": st["tile_bits"], "low_bits": st["low_bits"],
                "tune_seconds": st["tune_seconds"],
+               "device_draw_seconds": st["draw_seconds"],
+               "round_graph_builds": gb.value - g0[0],
+               "round_graph_replays": gr.value - g0[1],
                "launches_per_kernel": kernels,
                "fused_each_amp_updates_per_s":
                    sf["amp_updates"] / sf["seconds"]},

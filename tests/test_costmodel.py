@@ -1,6 +1,7 @@
 """Cost model: feature extraction (CPU), the HIP network against the numpy
 oracle (GPU), and the learned schedule's bit identity (GPU)."""
 
+import ctypes as C
 import time
 
 import numpy as np
@@ -121,58 +122,97 @@ def test_learned_schedule_bit_identical(oracle):
     assert misses >= 1 and hits >= misses   # second sweep reuses schedules
 
 
+def _time_sched(reg, xm, zm, phi, sched):
+    """Wall seconds of one block run under an explicit schedule."""
+    L = nat.lib()
+    L.ph2_reg_rotate_sched.restype = C.c_int
+    u32p = C.POINTER(C.c_uint32)
+    lens, tbs, lbs = (np.ascontiguousarray(a, np.uint32) for a in sched)
+    nat.synchronize()
+    t0 = time.perf_counter()
+    rc = L.ph2_reg_rotate_sched(
+        C.c_void_p(reg.h), C.c_size_t(len(xm)),
+        xm.ctypes.data_as(C.POINTER(C.c_uint64)),
+        zm.ctypes.data_as(C.POINTER(C.c_uint64)),
+        phi.ctypes.data_as(C.POINTER(C.c_double)),
+        lens.ctypes.data_as(u32p), tbs.ctypes.data_as(u32p),
+        lbs.ctypes.data_as(u32p), C.c_size_t(len(lens)))
+    nat.synchronize()
+    assert rc == 0
+    return time.perf_counter() - t0
+
+
 @pytest.mark.gpu
 def test_learned_schedule_gain_over_greedy():
     """The committed network's schedule against the library's default
-    greedy cut (tile 12, low 3) on 28-qubit sweep blocks: its predicted time
-    is never above the greedy cut's (the greedy passes are among the
-    candidates the dynamic program chooses from), and the measured times of
-    the two are reported.  The prediction errors are as large as the
-    differences between candidates, so the measured bound only catches a
-    schedule that is clearly worse.  The gains on held-out 33-qubit blocks
-    and on the composite integrator are recorded by
-    phase2_amd.train_costmodel."""
+    greedy cut (tile 12, low 3) on 30-qubit sweep blocks of a Hamiltonian
+    it was not trained on (generator seed 7).  Each block is timed under
+    both schedules, interleaved, REPS times after a warm-up; a block's
+    noise is the larger relative spread (max/min - 1) of its two series,
+    floored at 0.5%.  Where the network predicts a margin over greedy of
+    more than twice that noise, the learned schedule's best time must not
+    exceed greedy's by more than the noise; at least one block must be such
+    a block, and over all blocks the learned schedules must be faster.  The
+    predicted total is never above greedy's (the greedy passes are among
+    the candidates the dynamic program chooses from)."""
     nat.init()
     params, _ = cm.load()
-    n = 28
-    hm = synth.hamiltonian(n, 0, seed=7, decay=True, n_total=1024 * 8)
+    n, reps = 30, 5
+    hm = synth.hamiltonian(n, 0, seed=7, decay=True, n_total=1024 * 14)
     perm = nat.sort_lex(hm.xm, hm.zm)
     xm, zm, phi = hm.xm[perm], hm.zm[perm], 0.02 * hm.cf[perm]
-    blocks = [slice(b * 1024, (b + 1) * 1024) for b in (2, 4, 6)]
+    blocks = [(np.ascontiguousarray(xm[s]), np.ascontiguousarray(zm[s]),
+               np.ascontiguousarray(phi[s]))
+              for s in (slice(b * 1024, (b + 1) * 1024)
+                        for b in (2, 5, 8, 11))]
+    rows = []
     try:
         cm.install(params, min_qb=n)
-        pred_g = pred_l = 0.0
-        for s in blocks:
-            g = cm.schedule(xm[s], zm[s], n, greedy=(12, 3))
-            lr = cm.schedule(xm[s], zm[s], n)
+        scheds = []
+        for bx, bz, _ in blocks:
+            g = cm.schedule(bx, bz, n, greedy=(12, 3))
+            lr = cm.schedule(bx, bz, n)
             assert lr[0].sum() == 1024 and g[0].sum() == 1024
-            pred_g += cm.predict(xm[s], zm[s], n, g).sum()
-            pred_l += cm.predict(xm[s], zm[s], n, lr).sum()
-        assert pred_l <= pred_g * (1 + 1e-6)
+            pg = cm.predict(bx, bz, n, g).sum()
+            pl = cm.predict(bx, bz, n, lr).sum()
+            assert pl <= pg * (1 + 1e-6)
+            scheds.append((g, lr, pg, pl))
         reg = nat.Register(n, nat.FUSED, 12, 3)
-        reg.prep_multidet(np.array([0xFFF], np.uint64), np.array([1.0 + 0j]))
-        best = {}
-        for rep in range(3):
-            for mode in ("greedy", "learned"):
-                if mode == "greedy":
-                    cm.uninstall()
-                else:
-                    cm.install(params, min_qb=n)
-                nat.synchronize()
-                t0 = time.perf_counter()
-                for s in blocks:
-                    reg.rotate(xm[s], zm[s], phi[s])
-                nat.synchronize()
-                dt = time.perf_counter() - t0
-                best[mode] = min(best.get(mode, 1e9), dt)
-        reg.free()
+        try:
+            reg.prep_multidet(np.array([(1 << 15) - 1], np.uint64),
+                              np.array([1.0 + 0j]))
+            times = [([], []) for _ in blocks]
+            for rep in range(reps + 1):
+                for k, (bx, bz, bp) in enumerate(blocks):
+                    for m in (0, 1):
+                        dt = _time_sched(reg, bx, bz, bp, scheds[k][m])
+                        if rep:
+                            times[k][m].append(dt)
+        finally:
+            reg.free()
     finally:
         cm.uninstall()
-    print(f"predicted ms: greedy {pred_g:.1f} learned {pred_l:.1f}; "
-          f"measured s: greedy {best['greedy']:.4f} "
-          f"learned {best['learned']:.4f} "
-          f"gain {best['greedy'] / best['learned']:.4f}x")
-    assert best["learned"] <= 1.10 * best["greedy"]
+    checked = 0
+    tot_g = tot_l = 0.0
+    for k, (tg, tl) in enumerate(times):
+        tg, tl = np.array(tg), np.array(tl)
+        noise = max(tg.max() / tg.min() - 1, tl.max() / tl.min() - 1, 0.005)
+        g, lr, pg, pl = scheds[k]
+        margin = 1 - pl / pg
+        gain = tg.min() / tl.min()
+        tot_g += tg.min()
+        tot_l += tl.min()
+        rows.append((k, len(g[0]), len(lr[0]), margin, noise, gain))
+        print(f"block {k}: passes greedy {len(g[0])} learned {len(lr[0])}; "
+              f"predicted margin {margin:.4f}, noise {noise:.4f}, "
+              f"measured gain {gain:.4f}x", flush=True)
+        if margin > 2 * noise:
+            checked += 1
+            assert tl.min() <= tg.min() * (1 + noise), rows[-1]
+    print(f"total: greedy {tot_g:.4f} s, learned {tot_l:.4f} s, "
+          f"gain {tot_g / tot_l:.4f}x, blocks checked {checked}")
+    assert checked >= 1, rows
+    assert tot_l < tot_g, rows
 
 
 def test_load_rejects_other_feature_vectors(tmp_path):
