@@ -182,7 +182,9 @@ def roofline(kernel, prof):
     return {"bound": "hbm", "kernel": kernel, "achieved": gbs,
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
             "traffic": None, "algorithmic_bytes_per_launch": nbytes,
-            "mean_launch_ms": ms, "launches": prof["launches"]}
+            "mean_launch_ms": ms, "launches": prof["launches"],
+            "terms_per_launch": prof["terms"] / launches
+            if "terms" in prof else None}
 
 
 def traffic_for(kernel, nqb):
