@@ -62,6 +62,10 @@ METRIC = ("Pauli amp-updates/s + % HBM peak, 33-qb FP64 Trotter: "
           "fused/batched vs plain")
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_AMP_PASS = 32
+# FP64 multiplies and adds at full rate (MI355X: 78.6 TFLOP/s counts an FMA
+# as two; a rotation uses no FMA, so one operation per lane per cycle)
+FP64_PEAK_TOPS = 78.6 / 2
+OPS_PER_PAIR = 12   # 8 multiplies and 4 adds per amplitude pair and term
 WORKLOADS = ("trott_n33", "trott_n16", "trott2_n26", "qdrift_batch",
              "cmpsit_n30")
 DEFAULT_STEPS = dict(trott_n33=8, trott_n16=64, trott2_n26=4,
@@ -173,18 +177,34 @@ def lex(hm, nat):
     return hm.xm[perm], hm.zm[perm], hm.cf[perm]
 
 
-def roofline(kernel, prof):
-    """Roofline object of one profiled kernel kind."""
+def roofline(kernel, prof, rotations=True):
+    """Roofline object of one profiled kernel kind.  With rotations (the
+    records count rotations, not batched work items), also the FP64 side:
+    algorithmic multiplies and adds per launch against the full-rate peak,
+    and hbm_plus_fp64_frac = (HBM time at peak + FP64 time at peak) / the
+    mean launch time, which is 1 for a pass that runs both resources at
+    peak one after the other."""
     launches = max(1, prof["launches"])
     ms = prof["ms"] / launches
     nbytes = BYTES_PER_AMP_PASS * prof["amps"] / launches
     gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-    return {"bound": "hbm", "kernel": kernel, "achieved": gbs,
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-            "traffic": None, "algorithmic_bytes_per_launch": nbytes,
-            "mean_launch_ms": ms, "launches": prof["launches"],
-            "terms_per_launch": prof["terms"] / launches
-            if "terms" in prof else None}
+    out = {"bound": "hbm", "kernel": kernel, "achieved": gbs,
+           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+           "traffic": None, "algorithmic_bytes_per_launch": nbytes,
+           "mean_launch_ms": ms, "launches": prof["launches"],
+           "terms_per_launch": prof["terms"] / launches
+           if "terms" in prof and rotations else None}
+    if rotations and "terms" in prof and ms > 0:
+        # every term of a pass touches every amplitude pair once
+        ops = (OPS_PER_PAIR * (prof["amps"] / launches / 2)
+               * (prof["terms"] / launches))
+        tops = ops / (ms * 1e-3) / 1e12
+        out["fp64"] = {"ops_per_launch": ops, "achieved": tops,
+                       "peak": FP64_PEAK_TOPS,
+                       "unit": "TFLOP/s (multiply/add, no FMA)",
+                       "frac": tops / FP64_PEAK_TOPS}
+        out["hbm_plus_fp64_frac"] = out["frac"] + out["fp64"]["frac"]
+    return out
 
 
 def traffic_for(kernel, nqb):
@@ -583,7 +603,8 @@ def bench_qdrift_batch(args, nat, costmodel, schedule):
         plain={"amp_updates_per_s": sp["amp_updates"] / sp["seconds"],
                "subset": "first 64 registers, 1 sample, one launch per "
                "rotation", "launches": sp["launches"]},
-        roofline=roofline(kern or fused_name(st["tile_bits"], True), kp),
+        roofline=roofline(kern or fused_name(st["tile_bits"], True), kp,
+                          rotations=False),
         config={"workload": "qdrift_batch_512", "registers": len(regs),
                 "widths": [int(min(widths)), int(max(widths))],
                 "samples": args.steps, "depth": 1000,

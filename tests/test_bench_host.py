@@ -47,6 +47,16 @@ def test_dominant_fused_by_time_not_count():
     assert rf["algorithmic_bytes_per_launch"] == 32 * 2 ** 33
     assert rf["achieved"] == pytest.approx(32 * 2 ** 33 / 0.040 / 1e9)
     assert rf["frac"] == pytest.approx(rf["achieved"] / 8000.0)
+    # 20 terms per launch, each 12 multiplies/adds per amplitude pair
+    assert rf["terms_per_launch"] == 20
+    ops = 12 * 20 * 2 ** 32
+    assert rf["fp64"]["ops_per_launch"] == ops
+    assert rf["fp64"]["frac"] == pytest.approx(ops / 0.040 / 39.3e12)
+    assert rf["hbm_plus_fp64_frac"] == pytest.approx(
+        rf["frac"] + rf["fp64"]["frac"])
+    # batched records count work items, not rotations: no FP64 side
+    rb = bench.roofline(name, st, rotations=False)
+    assert "fp64" not in rb and rb["terms_per_launch"] is None
 
 
 def test_half_tile_form_is_its_own_kernel():
