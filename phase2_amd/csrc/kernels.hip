@@ -2867,6 +2867,8 @@ int ph2hip_sample_cdf(size_t nstream, const double *const *cdf,
 		return -1;
 	if (nstream == 0)
 		return 0;
+	if (nstream > (size_t)INT32_MAX)
+		return fail("sample_cdf: %zu streams", nstream);
 	size_t ntab = 0, nout = 0;
 	for (size_t k = 0; k < nstream; k++) {
 		if (len[k] == 0 || len[k] > UINT32_MAX || !cdf[k])
