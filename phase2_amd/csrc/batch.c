@@ -308,9 +308,8 @@ static int prep_round(struct batch *B, size_t r, size_t lim)
 		const struct breg *g = &B->b[i];
 		if (!starts_sample(g, r))
 			continue;
-		const uint64_t mask = (UINT64_C(1) << g->nqb) - 1;
 		for (size_t d = 0; d < g->ndet; d++) {
-			B->idx[m] = g->off + (g->didx[d] & mask);
+			B->idx[m] = g->off + g->didx[d];
 			B->val[2 * m] = g->dcf[2 * d];
 			B->val[2 * m + 1] = g->dcf[2 * d + 1];
 			m++;
@@ -443,9 +442,8 @@ static int measure_round(struct batch *B, size_t r, double *out)
 		const struct breg *g = &B->b[i];
 		if (!ends_sample(g, r))
 			continue;
-		const uint64_t mask = (UINT64_C(1) << g->nqb) - 1;
 		for (size_t d = 0; d < g->ndet; d++)
-			B->idx[m++] = g->off + (g->didx[d] & mask);
+			B->idx[m++] = g->off + g->didx[d];
 	}
 	if (m && ph2hip_gather_abs(B->amp, B->namp, B->idx, m, B->val) < 0)
 		return -1;
@@ -612,13 +610,23 @@ static int batch_alloc(struct batch *B, size_t n)
 	return B->b && B->jobs ? 0 : -1;
 }
 
-static int check_reg(size_t i, uint32_t nqb, size_t nterms)
+static int check_reg(size_t i, uint32_t nqb, size_t nterms, size_t ndet,
+	const uint64_t *didx)
 {
 	if (nqb == 0 || nqb > 40 || nterms == 0 || nterms > UINT32_MAX) {
 		log_error("batch: register %zu: nqb=%u nterms=%zu", i, nqb,
 			nterms);
 		return -1;
 	}
+	/* an index beyond the register would alias a lower one (or another
+	 * register of the shared buffer): refused, as for one register */
+	for (size_t d = 0; d < ndet; d++)
+		if (didx[d] >> nqb) {
+			log_error("batch: register %zu: determinant %zu index "
+				"%llu beyond %u qubits", i, d,
+				(unsigned long long)didx[d], nqb);
+			return -1;
+		}
 	return 0;
 }
 
@@ -631,7 +639,8 @@ int ph2_batch_qdrift(const struct ph2_batch_reg *regs, size_t nreg, int mode,
 	struct batch B;
 	int rc = batch_alloc(&B, nreg);
 	for (size_t i = 0; i < nreg && rc == 0; i++) {
-		rc = check_reg(i, regs[i].nqb, regs[i].nterms);
+		rc = check_reg(i, regs[i].nqb, regs[i].nterms, regs[i].ndet,
+			regs[i].didx);
 		if (rc == 0)
 			rc = reg_qdrift(&B.b[i], &regs[i]);
 	}
@@ -652,7 +661,8 @@ int ph2_batch_cmpsit(const struct ph2_batch_cmpsit_reg *regs, size_t nreg,
 	struct batch B;
 	int rc = batch_alloc(&B, nreg);
 	for (size_t i = 0; i < nreg && rc == 0; i++) {
-		rc = check_reg(i, regs[i].nqb, regs[i].nterms);
+		rc = check_reg(i, regs[i].nqb, regs[i].nterms, regs[i].ndet,
+			regs[i].didx);
 		if (rc == 0)
 			rc = reg_cmpsit(&B.b[i], &regs[i]);
 	}

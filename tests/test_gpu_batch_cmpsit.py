@@ -103,6 +103,24 @@ def test_batch_cmpsit_rejects_empty_sample_set():
         nat.batch_cmpsit([r], nat.BATCH_FUSED_EACH, 9, 2)
 
 
+@pytest.mark.parametrize("kind", ["qdrift", "cmpsit"])
+def test_batch_rejects_determinant_beyond_register(kind):
+    """A determinant index of 2^n or more is refused (it would alias a
+    lower index, or land in the next register of the shared buffer)."""
+    r = _cmpsit_regs([10, 11], steps=[1, 1], samples=[1, 1])
+    idx = np.array(r[1]["det_idx"], dtype=np.uint64)
+    idx[1] = np.uint64(1 << 11)
+    r[1]["det_idx"] = idx
+    if kind == "qdrift":
+        for x in r:
+            x["step_size"] = 0.05
+        with pytest.raises(RuntimeError, match="ph2_batch_qdrift"):
+            nat.batch_qdrift(r, nat.BATCH_FUSED_EACH, 9, 2)
+    else:
+        with pytest.raises(RuntimeError, match="ph2_batch_cmpsit"):
+            nat.batch_cmpsit(r, nat.BATCH_FUSED_EACH, 9, 2)
+
+
 def _graph_stats():
     import ctypes as C
     b, r = C.c_uint64(0), C.c_uint64(0)
