@@ -1,8 +1,9 @@
 """Golden outputs of the simul.h5 path (ph2run), from the reference's own
 build.
 
-Copies of the reference's test fixtures (test/data/*.h5; data files) are
-kept in tests/golden/h5/.  For each case below the reference's ph2run
+Copies of the reference's test fixtures (test/data/*.h5; data files), and
+of the Hamiltonian of its examples pipeline (examples/data/hamil.h5, kept as
+examples_hamil.h5), are kept in tests/golden/h5/.  For each case below the reference's ph2run
 (oracle/_ref/ph2run_ref, built from its sources by oracle/Makefile.ref) runs
 on a fresh copy, and the per-step values it writes are read back with
 h5dump (17 significant digits, so every double is exact) into
@@ -24,6 +25,7 @@ import numpy as np
 
 HERE = Path(__file__).resolve().parent
 REF_DATA = Path("/root/reference/test/data")
+EXAMPLE_DATA = Path("/root/reference/examples/data")
 PH2RUN = HERE.parents[1] / "oracle" / "_ref" / "ph2run_ref"
 H5DUMP = shutil.which("h5dump") or "/opt/conda/bin/h5dump"
 
@@ -61,7 +63,17 @@ CASES = [
      ["trott", "--delta", "0.3", "--steps", "2"]),
     ("csf_empty_rejected", "N4_csf_empty.h5", "circ_trott",
      ["trott", "--delta", "0.3", "--steps", "2"]),
+    # the examples pipeline as the reference's CI runs it
+    # (examples/simul/trott/Makefile, CI.yml: TROTT_STEPS=256)
+    ("example_trott256", "examples_hamil.h5", "circ_trott",
+     ["trott", "--delta", "0.1", "--steps", "256"]),
 ]
+
+
+def source(fixture):
+    if fixture == "examples_hamil.h5":
+        return EXAMPLE_DATA / "hamil.h5"
+    return REF_DATA / fixture
 
 
 def dump_values(path, grp):
@@ -108,13 +120,13 @@ def dump_all(path):
 
 
 def main():
-    for f in FIXTURES:
-        shutil.copyfile(REF_DATA / f, HERE / "h5" / f)
+    for f in FIXTURES + ["examples_hamil.h5"]:
+        shutil.copyfile(source(f), HERE / "h5" / f)
     res = {}
     with tempfile.TemporaryDirectory() as td:
         for name, fix, grp, args in CASES:
             p = Path(td) / f"{name}.h5"
-            shutil.copyfile(REF_DATA / fix, p)
+            shutil.copyfile(source(fix), p)
             r = subprocess.run([str(PH2RUN), "-S", str(p)] + args,
                                capture_output=True, text=True)
             res[f"{name}__status"] = np.array(0 if r.returncode == 0 else 1)

@@ -6,7 +6,10 @@ sources).  On copies of the reference's fixture files it writes the same
 per-step values as the reference's own CPU build did
 (tests/golden/ph2run_h5.npz): bit for bit for multideterminant states, to
 1e-12 relative for coefficient-matrix states (their overlap is a device
-tree reduction); files the reference rejects are rejected too."""
+tree reduction); files the reference rejects are rejected too.  The
+256-step series on the bundled Hamiltonian of the reference's examples
+pipeline also gives a ground-state energy inside the band the reference's
+CI asserts."""
 
 import ctypes as C
 import shutil
@@ -15,6 +18,9 @@ from pathlib import Path
 
 import numpy as np
 import pytest
+
+from pipeline_energy import BAND, fft_energy
+from test_example_pipeline import hamil_attrs
 
 ROOT = Path(__file__).resolve().parents[1]
 PH2RUN = ROOT / "oracle" / "_ref" / "ph2run_hip"
@@ -59,7 +65,13 @@ def test_ph2run_dropin(golden, tmp_path, case):
     assert r.returncode == 0, (r.stdout + r.stderr)[-2000:]
     got, want = _values(p, grp), g[f"{case}__values"]
     assert got.shape == want.shape
-    if fix in ("N4_multidet.h5", "H2O_CAS56.h5"):      # multidet states
-        assert np.array_equal(got, want), (got, want)
+    if fix in ("N4_multidet.h5", "H2O_CAS56.h5", "examples_hamil.h5"):
+        assert np.array_equal(got, want), (got, want)   # multidet states
     else:
         assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want))
+    if case == "example_trott256":
+        # the reference CI's acceptance of its examples pipeline
+        norm, offset = hamil_attrs(p)
+        e = fft_energy(got, 0.1, norm, offset)
+        assert BAND[0] < e < BAND[1], e
+        assert e == fft_energy(want, 0.1, norm, offset)
