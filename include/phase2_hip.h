@@ -46,9 +46,6 @@ int ph2hip_d2d(void *dst, const void *src, size_t bytes);
 
 /* Zero namp amplitudes with a fill kernel (phase2/qreg_cuda.c:96). */
 int ph2hip_zero(void *amp, uint64_t namp);
-/* Single-amplitude write/read (phase2/qreg_cuda.c:70,85). */
-int ph2hip_set1(void *amp, uint64_t i, double re, double im);
-int ph2hip_get1(const void *amp, uint64_t i, double *re_im);
 
 /* -- rotations ------------------------------------------------------------ */
 

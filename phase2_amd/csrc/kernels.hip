@@ -2368,17 +2368,6 @@ int ph2hip_zero(void *amp, uint64_t namp)
 	return 0;
 }
 
-int ph2hip_set1(void *amp, uint64_t i, double re, double im)
-{
-	const double v[2] = { re, im };
-	return ph2hip_h2d((double2 *)amp + i, v, sizeof v);
-}
-
-int ph2hip_get1(const void *amp, uint64_t i, double *re_im)
-{
-	return ph2hip_d2h(re_im, (const double2 *)amp + i, 2 * sizeof(double));
-}
-
 int ph2hip_rotate_plain(void *amp, uint32_t nqb, const struct ph2hip_rot *rot,
 	size_t n)
 {

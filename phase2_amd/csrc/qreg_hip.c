@@ -41,7 +41,27 @@ struct qreg_hip {
 	 * overlap plan of circ.c), released by qreg_free */
 	void *aux;
 	void (*aux_free)(void *);
+	/* qreg_setamp writes not yet on the device, in call order */
+	uint64_t *qidx;
+	double *qval;
+	size_t nq, qcap;
+	/* host copy of amplitudes [win_lo, win_lo + win_n) as the device
+	 * holds them with the queued writes applied; win_n = 0: none */
+	double *win;
+	uint64_t win_lo, win_n;
 };
+
+/* One amplitude moved by itself costs a synchronous device copy (about
+ * 10 µs), so a loop over a register's amplitudes through qreg_setamp /
+ * qreg_getamp, as the reference's callers and tests write, would run at
+ * 10^5 amplitudes per second.  Writes are queued on the host and land in
+ * one scatter (later writes to an index win) before anything else touches
+ * the device buffer; reads are served from a host copy of an aligned
+ * window of AMP_WIN amplitudes, taken after the queue has landed, kept
+ * current by later qreg_setamp calls and dropped by every operation that
+ * may change the buffer. */
+#define AMP_WIN (UINT64_C(1) << 14)  /* 256 KiB */
+#define AMP_QMAX ((size_t)1 << 20)   /* 24 MiB of queued writes */
 
 static int g_mode = -1;
 /* tile 0: auto_tile(); low 3 measured best of 3..5 for the greedy cut with
@@ -111,10 +131,32 @@ void ph2_sched_install(ph2_sched_fn fn, void *ctx)
 	g_sched_ctx = ctx;
 }
 
+/* Put the queued writes on the device and drop the read window: called
+ * before anything else reads or changes the device buffer. */
+static int settle(struct qreg_hip *h, uint64_t namp)
+{
+	h->win_n = 0;
+	if (h->nq == 0)
+		return 0;
+	const int rc = ph2hip_scatter_set(h->damp, namp, h->qidx, h->qval,
+		h->nq);
+	h->nq = 0;
+	if (rc < 0) {
+		h->err = -1;
+		log_error("qreg_setamp: %s", ph2hip_strerror());
+	}
+	return rc;
+}
+
+/* The caller may read or change the buffer through the pointer, so the
+ * queued writes land first and the read window goes. */
 void *qreg_hip_device_ptr(struct qreg *reg)
 {
 	struct qreg_hip *h = reg->backend;
-	return h ? h->damp : NULL;
+	if (!h)
+		return NULL;
+	settle(h, reg->namp);
+	return h->damp;
 }
 
 static void set_err(struct qreg *reg, const char *what)
@@ -127,8 +169,17 @@ static void set_err(struct qreg *reg, const char *what)
 
 int qreg_error(const struct qreg *reg)
 {
-	const struct qreg_hip *h = reg->backend;
-	return h ? h->err : -1;
+	struct qreg_hip *h = reg->backend;
+	if (!h)
+		return -1;
+	/* a queued write fails, if at all, when it lands: land it now */
+	if (h->nq) {
+		const uint64_t lo = h->win_lo, n = h->win_n;
+		settle(h, reg->namp);
+		h->win_lo = lo;
+		h->win_n = n;
+	}
+	return h->err;
 }
 
 void qreg_clear_error(struct qreg *reg)
@@ -209,6 +260,9 @@ void qreg_free(struct qreg *reg)
 		ph2hip_free(h->damp);
 		free(h->rot);
 		ph2_sched_free(&h->sched);
+		free(h->qidx);
+		free(h->qval);
+		free(h->win);
 		free(h);
 	}
 	free(reg->amp);
@@ -233,23 +287,76 @@ static int in_range(struct qreg *reg, uint64_t i, const char *what)
 void qreg_getamp(struct qreg *reg, uint64_t i, _Complex double *z)
 {
 	struct qreg_hip *h = reg->backend;
-	double v[2] = { 0.0, 0.0 };
-	if (in_range(reg, i, "qreg_getamp") && ph2hip_get1(h->damp, i, v) < 0)
-		set_err(reg, "qreg_getamp");
+	*z = 0.0;
+	if (!in_range(reg, i, "qreg_getamp"))
+		return;
+	if (!h->win_n || i - h->win_lo >= h->win_n) {
+		if (settle(h, reg->namp) < 0)
+			return;
+		if (!h->win && !(h->win = malloc(AMP_WIN * 16))) {
+			log_error("qreg_getamp: alloc");
+			h->err = -1;
+			return;
+		}
+		const uint64_t lo = i & ~(AMP_WIN - 1);
+		const uint64_t n = reg->namp - lo < AMP_WIN ? reg->namp - lo :
+							       AMP_WIN;
+		if (ph2hip_d2h(h->win, (const char *)h->damp + lo * 16,
+			    n * 16) < 0) {
+			set_err(reg, "qreg_getamp");
+			return;
+		}
+		h->win_lo = lo;
+		h->win_n = n;
+	}
+	const double *v = h->win + 2 * (i - h->win_lo);
 	*z = CMPLX(v[0], v[1]);
 }
 
 void qreg_setamp(struct qreg *reg, uint64_t i, _Complex double z)
 {
 	struct qreg_hip *h = reg->backend;
-	if (in_range(reg, i, "qreg_setamp") &&
-	    ph2hip_set1(h->damp, i, creal(z), cimag(z)) < 0)
-		set_err(reg, "qreg_setamp");
+	if (!in_range(reg, i, "qreg_setamp"))
+		return;
+	if (h->nq == h->qcap) {
+		if (h->qcap >= AMP_QMAX) {
+			/* the window already holds these writes: keep it */
+			const uint64_t lo = h->win_lo, n = h->win_n;
+			if (settle(h, reg->namp) < 0)
+				return;
+			h->win_lo = lo;
+			h->win_n = n;
+		} else {
+			const size_t cap = h->qcap ? 2 * h->qcap : 1024;
+			uint64_t *qi = realloc(h->qidx, sizeof *qi * cap);
+			if (qi)
+				h->qidx = qi;
+			double *qv = realloc(h->qval, 2 * sizeof *qv * cap);
+			if (qv)
+				h->qval = qv;
+			if (!qi || !qv) {
+				log_error("qreg_setamp: alloc");
+				h->err = -1;
+				return;
+			}
+			h->qcap = cap;
+		}
+	}
+	h->qidx[h->nq] = i;
+	h->qval[2 * h->nq] = creal(z);
+	h->qval[2 * h->nq + 1] = cimag(z);
+	h->nq++;
+	if (h->win_n && i - h->win_lo < h->win_n) {
+		h->win[2 * (i - h->win_lo)] = creal(z);
+		h->win[2 * (i - h->win_lo) + 1] = cimag(z);
+	}
 }
 
 void qreg_zero(struct qreg *reg)
 {
 	struct qreg_hip *h = reg->backend;
+	h->nq = 0; /* every queued write would be overwritten */
+	h->win_n = 0;
 	if (ph2hip_zero(h->damp, reg->namp) < 0)
 		set_err(reg, "qreg_zero");
 }
@@ -263,6 +370,8 @@ void qreg_sync_host_to_device(struct qreg *reg)
 		h->err = -1;
 		return;
 	}
+	h->nq = 0; /* every queued write would be overwritten */
+	h->win_n = 0;
 	if (ph2hip_h2d(h->damp, reg->amp, reg->namp * 16) < 0)
 		set_err(reg, "qreg_sync_host_to_device");
 }
@@ -276,6 +385,8 @@ void qreg_sync_device_to_host(struct qreg *reg)
 		h->err = -1;
 		return;
 	}
+	if (settle(h, reg->namp) < 0)
+		return;
 	if (ph2hip_d2h(reg->amp, h->damp, reg->namp * 16) < 0)
 		set_err(reg, "qreg_sync_device_to_host");
 }
@@ -290,6 +401,8 @@ void qreg_paulirot(struct qreg *reg, struct paulis code_hi,
 		return;
 	}
 	if (ncodes == 0)
+		return;
+	if (settle(h, reg->namp) < 0)
 		return;
 	if (ncodes > h->rot_cap) {
 		struct ph2hip_rot *r = realloc(h->rot, sizeof *r * ncodes);
