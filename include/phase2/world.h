@@ -21,7 +21,12 @@ enum world_stat {
 	WORLD_ERR = 2,
 };
 
+/* The backend number of a build (0 qreg, 2 CUDA in the reference; 3 HIP);
+ * a consumer may set it on the command line, as the reference's headers
+ * allow. */
+#ifndef PHASE2_BACKEND
 #define PHASE2_BACKEND 3
+#endif
 #define WORLD_BACKEND "HIP"
 
 struct world_info {

@@ -35,6 +35,7 @@ FIXTURES = ["N4_multidet.h5", "N4_closed.h5", "N4_open.h5", "N4_csf.h5",
 # fixtures that only the reference's own t-data_* tests read (copied, not
 # dumped)
 COPY_ONLY = ["case-d9f603dc.h5_solved"]
+COPY_TREES = ["bendazzoli"]
 
 # (name, fixture, group, ph2run arguments after -S FILE)
 CASES = [
@@ -125,6 +126,8 @@ def dump_all(path):
 def main():
     for f in FIXTURES + COPY_ONLY + ["examples_hamil.h5"]:
         shutil.copyfile(source(f), HERE / "h5" / f)
+    for d in COPY_TREES:
+        shutil.copytree(REF_DATA / d, HERE / "h5" / d, dirs_exist_ok=True)
     res = {}
     with tempfile.TemporaryDirectory() as td:
         for name, fix, grp, args in CASES:
