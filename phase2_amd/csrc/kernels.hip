@@ -651,14 +651,27 @@ __device__ __forceinline__ void tile_groups(v2d *__restrict__ a, v2d *tile,
 }
 
 // Slot of tile coordinate v in a buffer of half a tile: bits h and TB-1
-// exchanged, the whole-tile slot map, bit TB-1 dropped.  Linear, and one-
-// to-one on each half {v : bit h of v = const}.
+// exchanged, then bits 4 and up folded into bits 0..3 through
+// y ^ (y >> 1) ^ (y >> 3) with y = v >> 4, then bit TB-1 dropped.  Linear,
+// and one-to-one on each half {v : bit h of v = const}.
+//
+// Banks, as measured on gfx950 with the LDS conflict counters: a 16-byte
+// write serves 8 lanes per clock (lane bits 0..2) over 32 banks, so those
+// lanes need distinct slots mod 8; a 16-byte read serves 16 lanes per
+// clock (lane bits 0..3) over 64 banks, distinct slots mod 16.  The fold
+// gives tile bits 4..10 the bank vectors 1, 3, 6, 0xd, 0xa, 4, 8 (bits 11
+// and 12 none), so that the host (half_bank) finds, for nearly every
+// group, three free lane bits independent mod 8 and a fourth independent
+// mod 16.  The whole-tile fold (bank vector t mod 4 for tile bit t) runs
+// out of a residue for many groups: 15% of LDS cycles were conflicts at
+// tile 12 with it, 6% at tile 13.
 template <int TB>
 __device__ __forceinline__ uint32_t half_slot(uint32_t v, uint32_t h)
 {
 	const uint32_t t = ((v >> h) ^ (v >> (TB - 1))) & 1u;
 	v ^= t * ((1u << h) | (1u << (TB - 1)));
-	return lds_slot(v) & ((1u << (TB - 1)) - 1);
+	v ^= ((v >> 4) ^ (v >> 5) ^ (v >> 7)) & 15u;
+	return v & ((1u << (TB - 1)) - 1);
 }
 
 // Byte offsets in the half buffer of a group's basis vectors (uniform).
@@ -1626,6 +1639,35 @@ int plain_one(double2 *a, uint32_t nqb, const struct ph2hip_rot &r)
 // for messages) in tiles of 2^tb amplitudes.  0, or -1 if the pass does not
 // fit its tile; with half set, 1 if the pass has no half-tile form (some
 // exchange between consecutive groups has no split bit).
+// Rank over GF(2) of v[0..n) masked by m.
+uint32_t gf2_rank(const uint32_t *v, uint32_t n, uint32_t m)
+{
+	uint32_t b[32], r = 0;
+	for (uint32_t k = 0; k < n; k++) {
+		uint32_t x = v[k] & m;
+		for (uint32_t i = 0; i < r && x; i++)
+			if (x >> (31 - __builtin_clz(b[i])) & 1)
+				x ^= b[i];
+		if (!x)
+			continue;
+		const int p = 31 - __builtin_clz(x);
+		for (uint32_t i = 0; i < r; i++)
+			if (b[i] >> p & 1)
+				b[i] ^= x;
+		b[r++] = x;
+	}
+	return r;
+}
+
+// Bank vector (bits 0..3 of the 16-byte slot) of tile bit t in the
+// half-tile buffer: half_slot's fold of bit t (bits 0..3 map to
+// themselves).
+uint32_t half_bank(uint32_t t)
+{
+	const uint32_t v = 1u << t;
+	return (v ^ (v >> 4) ^ (v >> 5) ^ (v >> 7)) & 15u;
+}
+
 int build_pass_mode(FusedPass &P, const struct ph2hip_rot *r, size_t len,
 	uint32_t tb, uint32_t low_bits, size_t k0, bool half)
 {
@@ -1791,10 +1833,17 @@ int build_pass_mode(FusedPass &P, const struct ph2hip_rot *r, size_t len,
 			G.piv[i] = (uint8_t)(31 - __builtin_clz(u[i]));
 		}
 		// Coset index bits -> free tile bits.  The first four (lanes
-		// 0..15 of a wave) take free bits whose LDS-slot images
-		// (tile bit t < 12 lands on slot bit t mod 4) are distinct,
-		// so a quarter-wave reads 16 different bank groups.  Reserved
-		// index bits take their exchange split bits.
+		// 0..15 of a wave) take free bits whose bank vectors are
+		// independent.  In the whole-tile form tile bit t < 12 lands
+		// on bank bit t mod 4, one bit per residue.  In the half-tile
+		// form the bank vector is half_bank(t): the first three lanes
+		// (served together by a 16-byte write) take bits independent
+		// mod 8, the fourth one independent mod 16 (a 16-byte read
+		// serves 16 lanes), in ascending order: tile bits 0..3 first
+		// where free, whole 256-byte runs for the groups that move
+		// their cosets to and from HBM; tile bit tb - 1, which the
+		// exchange moves to the split bit, last.
+		// Reserved index bits take their exchange split bits.
 		{
 			uint32_t used = rbits, npos = 0;
 			for (uint32_t i = 0; i < D; i++)
@@ -1805,15 +1854,38 @@ int build_pass_mode(FusedPass &P, const struct ph2hip_rot *r, size_t len,
 					G.jpos[rpos[r]] = (uint8_t)rbit[r];
 					taken[rpos[r]] = true;
 				}
-			for (uint32_t res = 0; res < 4; res++)
-				for (uint32_t t = res; t < tb && t < 12; t += 4)
-					if (!(used >> t & 1)) {
-						used |= 1u << t;
+			if (half) {
+				// round 0: three lanes independent mod 8;
+				// round 1: a fourth independent mod 16;
+				// round 2: any free bits if those do not exist
+				uint32_t bv[5], nl = 0;
+				for (int round = 0; round < 3 && nl < 4; round++)
+					for (uint32_t tt = 0; tt < tb &&
+						nl < (round == 0 ? 3u : 4u); tt++) {
+						if (used >> tt & 1)
+							continue;
+						bv[nl] = half_bank(tt);
+						const uint32_t m = round == 0 ? 7u : 15u;
+						if (round < 2 &&
+						    gf2_rank(bv, nl + 1, m) != nl + 1)
+							continue;
+						used |= 1u << tt;
 						while (taken[npos])
 							npos++;
-						G.jpos[npos++] = (uint8_t)t;
-						break;
+						G.jpos[npos++] = (uint8_t)tt;
+						nl++;
 					}
+			} else {
+				for (uint32_t res = 0; res < 4; res++)
+					for (uint32_t t = res; t < tb && t < 12; t += 4)
+						if (!(used >> t & 1)) {
+							used |= 1u << t;
+							while (taken[npos])
+								npos++;
+							G.jpos[npos++] = (uint8_t)t;
+							break;
+						}
+			}
 			for (uint32_t t = 0; t < tb; t++)
 				if (!(used >> t & 1)) {
 					used |= 1u << t;

@@ -93,15 +93,14 @@ def _groups(xm, zm, tile, low):
                  count=int(r[19])) for r in a]
 
 
-def _lds_slot(v):
-    return v ^ (((v >> 4) ^ (v >> 8)) & 15)
-
-
 def _half_slot(v, h, tb):
-    # the kernel's map: bits h and tb-1 exchanged, slot map, bit tb-1 dropped
+    # the kernel's map: bits h and tb-1 exchanged, bits 4 and up folded
+    # into bits 0..3 by y ^ (y >> 1) ^ (y >> 3) (y = v >> 4), bit tb-1
+    # dropped
     t = ((v >> h) ^ (v >> (tb - 1))) & 1
     v ^= t * ((1 << h) | (1 << (tb - 1)))
-    return _lds_slot(v) & ((1 << (tb - 1)) - 1)
+    v ^= ((v >> 4) ^ (v >> 5) ^ (v >> 7)) & 15
+    return v & ((1 << (tb - 1)) - 1)
 
 
 def _coset(g, j, w, tb, d):
@@ -154,3 +153,55 @@ def test_half_tile_exchange_moves_every_amplitude(n, seed, low, tile):
                     assert table[(half, _half_slot(v, h, tile))] == v
             checked += 1
     assert checked > 0
+
+
+def _exchange_conflicts(n, seed, low, tile, slot):
+    """Lane groups served together in the exchanges of half-tile passes,
+    and how many of them meet a bank conflict, by the bank rules measured
+    on gfx950 with the LDS conflict counters: a 16-byte write serves lanes
+    0..7 of a group of eight (lane bits 0..2) per clock over 32 banks
+    (slots distinct mod 8), a 16-byte read lanes 0..15 (lane bits 0..3)
+    over 64 banks (distinct mod 16).  Group q writes with its layout and
+    group q+1 reads with its own.  The register offset of an access is
+    uniform, so register 0 stands for all 16."""
+    d = 4
+    nthreads = 1 << (tile - d)
+    total = bad = 0
+    for xm, zm in _passes(n, seed, tile, low, maxlen=40):
+        rc, sh = _shape(xm, zm, tile, low)
+        if rc != 0 or not sh[3] & 4 or sh[2] < 2:
+            continue
+        gs = _groups(xm, zm, tile, low)
+        for q in range(len(gs) - 1):
+            h = gs[q]["xh"]
+            for g, lanes, banks in ((gs[q], 8, 8), (gs[q + 1], 16, 16)):
+                for j0 in range(0, nthreads, lanes):
+                    sl = {slot(_coset(g, j, 0, tile, d), h, tile)
+                          for j in range(j0, j0 + lanes)}
+                    total += 1
+                    bad += len({x % banks for x in sl}) < len(sl)
+    return total, bad
+
+
+def _whole_fold_half_slot(v, h, tb):
+    # the fold of the whole-tile form (bank bit t mod 4 for tile bit t < 12)
+    t = ((v >> h) ^ (v >> (tb - 1))) & 1
+    v ^= t * ((1 << h) | (1 << (tb - 1)))
+    v ^= ((v >> 4) ^ (v >> 8)) & 15
+    return v & ((1 << (tb - 1)) - 1)
+
+
+@pytest.mark.parametrize("n,seed,low,tile", [(24, 21, 3, 12),
+                                             (30, 22, 3, 13)])
+def test_half_tile_exchange_spreads_lanes_over_banks(n, seed, low, tile):
+    """The lanes served together take coset-index bits on free tile bits
+    whose bank vectors under the half-tile fold are independent (three
+    mod 8 for writes, four mod 16 for reads), so nearly every lane group of
+    an exchange is conflict-free; the same layouts under the whole-tile
+    fold are not."""
+    total, bad = _exchange_conflicts(n, seed, low, tile, _half_slot)
+    assert total > 1000
+    assert bad <= 0.03 * total, (bad, total)
+    total_w, bad_w = _exchange_conflicts(n, seed, low, tile,
+                                         _whole_fold_half_slot)
+    assert bad_w > 3 * bad + 0.02 * total_w, (bad_w, bad, total_w)
