@@ -334,7 +334,8 @@ def bench_trott_n33(args, nat, costmodel, schedule):
 
     # Warm-up.  With the learned schedule every timed block is visited once
     # first, so its schedule is computed and cached as in steps 2.. of a
-    # Trotter run; the one-off scheduling time is reported separately.
+    # Trotter run.  These first visits (scheduling plus one sweep of each
+    # block) are timed apart from the steps, as first_visit_seconds.
     sched_s = 0.0
     nwarm = max(args.warmup, len(blocks) if schedule == "learned" else 0)
     for k in range(nwarm):
