@@ -246,10 +246,14 @@ def plain_rate(nat, nqb, xm, zm, phi):
                 diag_launches=pp["plain_diag"]["launches"])
 
 
-def auto_tile(tile, nqb):
-    """The library's tile width for a requested one (0: automatic)."""
+def auto_tile(tile, nqb, replay_max_qb=20):
+    """The library's tile width for a requested one (0: automatic), with
+    graph replay of repeated sweeps on for registers of at most
+    replay_max_qb qubits (the library's default)."""
     if tile:
         return tile
+    if nqb <= replay_max_qb:
+        return max(9, min(11 if nqb >= 19 else 10, nqb - 2))
     return max(9, 13 if nqb >= 26 else 12 if nqb > 16 else nqb - 5)
 
 

@@ -131,6 +131,8 @@ int ph2hip_set_fused_prefetch(uint32_t tile_bits, uint32_t amps,
  * reference. */
 int ph2hip_set_replay(uint32_t max_qb);
 uint64_t ph2hip_replay_hits(void);
+/* The current limit (replay on for registers of at most this many qubits). */
+uint32_t ph2hip_replay_max_qb(void);
 
 
 /* Shape of the fused pass over rot[0..n) without launching it (shape has 4

@@ -2516,13 +2516,8 @@ int ph2hip_rotate_fused(void *amp, uint32_t nqb, const struct ph2hip_rot *rot,
 			return fail("rotate_fused: term %zu flips qubit >= %u", k,
 				nqb);
 
-	if (!g_replay_init) {
-		const char *e = getenv("PHASE2_HIP_REPLAY_QB");
-		if (e)
-			g_replay_max_qb = (uint32_t)strtoul(e, nullptr, 0);
-		g_replay_init = true;
-	}
 	ReplaySlot *slot = nullptr;
+	ph2hip_replay_max_qb();
 	if (nqb <= g_replay_max_qb && !g_prof && n > 0) {
 		const uint32_t *const sch[3] = { pass_len, tile_bits, low_bits };
 		bool seen = false;
@@ -2593,6 +2588,17 @@ int ph2hip_rotate_fused(void *amp, uint32_t nqb, const struct ph2hip_rot *rot,
 			replay_clear(*slot);
 	}
 	return rc;
+}
+
+uint32_t ph2hip_replay_max_qb(void)
+{
+	if (!g_replay_init) {
+		const char *e = getenv("PHASE2_HIP_REPLAY_QB");
+		if (e)
+			g_replay_max_qb = (uint32_t)strtoul(e, nullptr, 0);
+		g_replay_init = true;
+	}
+	return g_replay_max_qb;
 }
 
 int ph2hip_set_replay(uint32_t max_qb)

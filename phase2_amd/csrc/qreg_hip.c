@@ -71,16 +71,32 @@ static uint32_t g_tile = 0, g_low = 3;
 static ph2_sched_fn g_sched_fn;
 static void *g_sched_ctx;
 
-/* Tile width when none is set: 13 from 26 qubits (the half-tile form at
- * tile 13 sweeps 26 qubits 4.7% faster than at tile 12, and a pass still
- * has 8192 tiles), 12 from 17, nqb - 5 below.  Small registers are
- * latency-bound (a few dozen workgroups per pass); at 16 qubits a whole
- * Trotter run takes 0.476 ms per step at tile 11, 0.503 at 10, 0.532 at 9
- * and 0.564 at 12. */
+/* Tile width when none is set.
+ *
+ * Registers whose repeated sweeps replay from a captured graph (at most
+ * ph2hip_replay_max_qb() qubits, 20 by default) are bound by each pass's
+ * latency, not by launches.  Measured per step of a lex-ordered 470-611
+ * term sweep replayed 64 times, tile 10 is best from 12 to 18 qubits and
+ * tile 11 at 19 and 20.  Against the rule below, at 14, 16, 18 and 20
+ * qubits: 0.287 vs 0.322, 0.318 vs 0.339, 0.390 vs 0.507 and 0.696 vs
+ * 0.765 ms.  Never above nqb - 2.
+ *
+ * Otherwise: 13 from 26 qubits (the half-tile form at tile 13 sweeps 26
+ * qubits 4.7% faster than at tile 12, and a pass still has 8192 tiles), 12
+ * from 17, nqb - 5 below.  Small registers are latency-bound (a few dozen
+ * workgroups per pass); without replay, at 16 qubits a whole Trotter run
+ * takes 0.476 ms per step at tile 11, 0.503 at 10, 0.532 at 9 and 0.564 at
+ * 12. */
 static uint32_t auto_tile(uint32_t nqb)
 {
-	const uint32_t t = nqb >= 26 ? 13 : nqb > 16 ? 12 :
-		(nqb > 5 ? nqb - 5 : 0);
+	uint32_t t;
+	if (nqb <= ph2hip_replay_max_qb()) {
+		t = nqb >= 19 ? 11 : 10;
+		if (t + 2 > nqb)
+			t = nqb > 2 ? nqb - 2 : 0;
+	} else {
+		t = nqb >= 26 ? 13 : nqb > 16 ? 12 : (nqb > 5 ? nqb - 5 : 0);
+	}
 	return t < PH2HIP_TILE_MIN ? PH2HIP_TILE_MIN : t;
 }
 

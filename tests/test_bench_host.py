@@ -24,6 +24,17 @@ def test_fused_name():
     assert bench.fused_name(12, half=True) == "k_rot_fused_half<12, 4>"
 
 
+def test_auto_tile_mirrors_the_library_rule():
+    # replayed registers (<= 20 qubits by default): tile 10, 11 from 19,
+    # never above nqb - 2; wider: 12 from 17, 13 from 26
+    assert [bench.auto_tile(0, n) for n in (10, 11, 12, 16, 18, 19, 20)] \
+        == [9, 9, 10, 10, 10, 11, 11]
+    assert [bench.auto_tile(0, n) for n in (21, 25, 26, 33)] == \
+        [12, 12, 13, 13]
+    assert bench.auto_tile(0, 16, replay_max_qb=0) == 11
+    assert bench.auto_tile(12, 16) == 12
+
+
 def _recs(rows):
     r = np.zeros(len(rows), dtype=_native.REC_DTYPE)
     for i, (kind, nqb, tb, nt, ms) in enumerate(rows):
